@@ -1,0 +1,200 @@
+// vb_fft.hpp -- device-side building blocks of the 3-D real FFT for CDNA4.
+//
+// One 1-D transform of length N is carried by TF threads ("a lane"); each
+// thread holds E = N / TF elements in registers at positions j + TF*e
+// (j = thread index inside the lane, e = 0..E-1).  Every Stockham pass reads
+// its inputs at exactly those positions, so the first pass takes its data
+// straight from global memory, the last pass leaves its results in the same
+// positions (ready for coalesced stores), and the passes in between exchange
+// data through LDS.  Radices are 2, 3, 4 and 8; the radix plan per length is
+// fixed at compile time so every loop unrolls.
+//
+// Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host
+// in extended precision and rounded once to the storage type.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vb {
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mk(T a, T b) {
+  cpx<T> r; r.x = a; r.y = b; return r;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> scale(cpx<T> a, T s) { return mk(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cpx<T> conj(cpx<T> a) { return mk(a.x, -a.y); }
+// multiply by S*i (S = -1 forward, +1 inverse)
+template <int S, typename T>
+__device__ __forceinline__ cpx<T> rot(cpx<T> a) {
+  return S < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// radix plans: E elements per thread, TF threads per transform, radices
+// ---------------------------------------------------------------------------
+template <int N> struct Plan;
+template <> struct Plan<16>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
+template <> struct Plan<32>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 4}; };
+template <> struct Plan<48>  { static constexpr int E = 12, NP = 3; static constexpr int R[3] = {4, 4, 3}; };
+template <> struct Plan<64>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 8}; };
+template <> struct Plan<96>  { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 2, 3}; };
+template <> struct Plan<128> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 4, 4}; };
+template <> struct Plan<192> { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 4, 3}; };
+template <> struct Plan<256> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 4}; };
+template <> struct Plan<384> { static constexpr int E = 12, NP = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
+template <> struct Plan<512> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+
+template <int N> constexpr int TFof() { return N / Plan<N>::E; }
+
+template <int N, int P>
+constexpr int ns_before() {
+  int ns = 1;
+  for (int i = 0; i < P; ++i) ns *= Plan<N>::R[i];
+  return ns;
+}
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers
+// ---------------------------------------------------------------------------
+template <int R, int S, typename T> struct Dft;
+
+template <int S, typename T> struct Dft<2, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    cpx<T> a = v[0], b = v[1];
+    v[0] = a + b; v[1] = a - b;
+  }
+};
+
+template <int S, typename T> struct Dft<3, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    const T h = T(0.86602540378443864676372317075294);   // sqrt(3)/2
+    cpx<T> t1 = v[1] + v[2];
+    cpx<T> t2 = mk(v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y);
+    cpx<T> d = scale(v[1] - v[2], h);
+    cpx<T> r = rot<S>(d);
+    v[0] = v[0] + t1;
+    v[1] = t2 + r;
+    v[2] = t2 - r;
+  }
+};
+
+template <int S, typename T> struct Dft<4, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    cpx<T> a = v[0] + v[2], b = v[0] - v[2];
+    cpx<T> c = v[1] + v[3], d = rot<S>(v[1] - v[3]);
+    v[0] = a + c; v[2] = a - c;
+    v[1] = b + d; v[3] = b - d;
+  }
+};
+
+template <int S, typename T> struct Dft<8, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    const T c = T(0.70710678118654752440084436210485);
+    cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, S, T>::run(e);
+    Dft<4, S, T>::run(o);
+    // W8^1 = (1 + S i) c, W8^2 = S i, W8^3 = (-1 + S i) c
+    cpx<T> o1 = scale(o[1] + rot<S>(o[1]), c);
+    cpx<T> o2 = rot<S>(o[2]);
+    cpx<T> o3 = scale(rot<S>(o[3]) - o[3], c);
+    v[0] = e[0] + o[0]; v[4] = e[0] - o[0];
+    v[1] = e[1] + o1;   v[5] = e[1] - o1;
+    v[2] = e[2] + o2;   v[6] = e[2] - o2;
+    v[3] = e[3] + o3;   v[7] = e[3] - o3;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Stockham passes.  Lds is a functor: lds(pos) -> cpx<T>& for this lane.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int S, int P>
+struct Pass {
+  template <class Lds>
+  static __device__ __forceinline__ void run(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
+    constexpr int E = Plan<N>::E;
+    constexpr int TF = N / E;
+    constexpr int R = Plan<N>::R[P];
+    constexpr int NS = ns_before<N, P>();
+    constexpr int ER = E / R;
+    constexpr bool last = (P == Plan<N>::NP - 1);
+    constexpr int TST = N / (NS * R);
+    cpx<T> y[last ? E : 1];
+#pragma unroll
+    for (int t = 0; t < ER; ++t) {
+      const int b = j + TF * t;
+      const int k = b % NS;
+      cpx<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = x[t + r * ER];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cpx<T> w = tw[r * k * TST];
+          if constexpr (S > 0) w.y = -w.y;
+          v[r] = cmul(v[r], w);
+        }
+      }
+      Dft<R, S, T>::run(v);
+      if constexpr (last) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) y[t + q * ER] = v[q];
+      } else {
+        const int base = (b / NS) * NS * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds(base + q * NS) = v[q];
+      }
+    }
+    if constexpr (last) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = y[e];
+    } else {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) x[e] = lds(j + TF * e);
+      __syncthreads();
+      Pass<T, N, S, P + 1>::run(x, j, tw, lds);
+    }
+  }
+};
+
+// Full 1-D transform (unnormalised): S = -1 forward, +1 inverse.
+// Contains (NP - 1) * 2 workgroup barriers; every thread of the workgroup
+// must call it the same number of times.
+template <typename T, int N, int S, class Lds>
+__device__ __forceinline__ void fft(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
+  Pass<T, N, S, 0>::run(x, j, tw, lds);
+}
+
+// LDS view for a lane whose transform is stored contiguously (x kernels):
+// one pad slot every 8 entries keeps stride-8 Stockham writes conflict-free.
+template <typename T>
+struct RowLds {
+  cpx<T>* base;
+  __device__ __forceinline__ cpx<T>& operator()(int pos) const { return base[pos + (pos >> 3)]; }
+};
+template <int N> constexpr int row_lds_len() { return N + N / 8; }
+
+// LDS view for column tiles (y and z kernels): TB interleaved transforms,
+// column index fastest.
+template <typename T, int TB>
+struct ColLds {
+  cpx<T>* base;  // already offset by the column index
+  __device__ __forceinline__ cpx<T>& operator()(int pos) const { return base[pos * TB]; }
+};
+
+}  // namespace vb

@@ -1,0 +1,72 @@
+// vb_host.hpp -- host-side internals of libvesicle (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vesicle.h"
+#include "vb_device.hpp"
+
+namespace vb {
+
+// Every HIP call is checked; a failure becomes VB_ERR_HIP with a message.
+struct HipError {
+  hipError_t err;
+  std::string where;
+};
+#define VB_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) throw ::vb::HipError{e_, std::string(#call) + " @ " +   \
+                                                  std::to_string(__LINE__)};      \
+  } while (0)
+
+void set_last_error(const std::string& msg);
+
+// ---- host tables (vb_lut.cpp, compiled without FMA contraction) -----------
+// L^ of Calc::init_fEIF (Calc.cu:599-600) for c = -m (2 pi / L)^2.
+double lhat(double m, double kfac2, double eps, double kappa, double D);
+// Coefficient records for every m in [0, mmax] (see Coef).
+void build_lut(std::vector<Coef>& out, int mmax, int scheme, int mode, double h, double kfac2, double eps,
+               double kappa, double D);
+// exp(-2 pi i m / n), m in [0, n), as interleaved doubles
+void build_twiddles(std::vector<double>& out, int n);
+int mmax_of(int nx, int ny, int nz);
+
+// ---- device buffers ---------------------------------------------------------
+template <typename X>
+struct DBuf {
+  X* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    free();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(X));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError{e, "hipMalloc"};
+    }
+    n = count;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { free(); }
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+};
+
+Geo make_geo(int nx, int nz, double L);
+
+// finalize kernel launcher (vb_solver.hip)
+void launch_finalize(const double* xpart, int nxb, const double* zpart, int nzb, Scal* sc, const Geo& g,
+                     const Phys& ph, int mode, hipStream_t s);
+void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s);
+
+}  // namespace vb

@@ -1,0 +1,678 @@
+// vb_solver.hip -- fused stage pipeline and drivers of one vesicle solve.
+//
+// State: the half spectrum u^ of the phase field (two buffers, current and
+// candidate, so a rejected adaptive step needs no copy-back).  Work buffers:
+// P, Q (spectra of the stage field and of its Laplacian during the inverse
+// transform), Tt (tmp1 / N pipeline), W (f pipeline, Euler only), pre and acc
+// (ETD combine accumulators), v and f (real stage field and f), block partial
+// sums, and the device scalars (V, A, E_b, E, lambda1, lambda2, alpha, beta).
+//
+// A "prefix" turns the prepared stage field into v, f, lap tmp1 and the
+// integrals; a "suffix" forms N, its spectrum and the ETD combine of one
+// stage, and prepares the next stage field.  One ETDRK4 evaluation is
+// prefix(1) suffix(1) ... prefix(4) suffix(4), where prefix(1) of the next
+// evaluation is also the energy evaluation of the new state.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "vb_solver.hpp"
+
+namespace vb {
+
+// ---------------------------------------------------------------------------
+// finalize: block partials -> V, A, E_b, E, lambda (Calc.cu:191-263,
+// phasefield.cpp:283-325, belastic.cpp:68-86 and 473-479)
+// ---------------------------------------------------------------------------
+struct FinArgs {
+  const double* xpart;
+  const double* zpart;
+  Scal* sc;
+  int nxb, nzb, mode, pad;
+  Geo g;
+  Phys ph;
+};
+
+__global__ void __launch_bounds__(256) k_finalize(FinArgs a) {
+  __shared__ double red[4 * 4];
+  __shared__ double tot[4];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < a.nxb; i += 256) {
+    s[0] += a.xpart[3 * i];
+    s[1] += a.xpart[3 * i + 1];
+    s[2] += a.xpart[3 * i + 2];
+  }
+  for (int i = threadIdx.x; i < a.nzb; i += 256) s[3] += a.zpart[i];
+  block_sum_store<4, 256>(s, red, tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Scal& c = *a.sc;
+    const double N = double(a.g.nr);
+    const double L3 = a.g.L3;
+    const double eps = a.ph.eps;
+    // vol: intf(d, 1, 1/2); surface: intf(s, 0, 3/(2 sqrt2)) with
+    // s = eps/2 |grad d|^2 + (d^2-1)^2/(4 eps); elastic: intf(f^2, 0, 1/(2 eps))
+    const double V = (tot[0] / N + 1.0) * (0.5 * L3);
+    const double ms = 0.5 * eps * tot[3] + 0.25 / eps * (tot[1] / N);
+    const double A = ms * (3.0 / (2.0 * sqrt(2.0)) * L3);
+    const double Eb = (tot[2] / N) * (1.0 / (eps * 2) * L3);
+    double alpha = c.alpha, beta = c.beta;
+    if (a.mode == FIN_DRIFT) {
+      alpha -= (V - c.alpha0);
+      beta -= (A - c.beta0);
+      c.alpha = alpha;
+      c.beta = beta;
+    }
+    const double ve = V - alpha, se = A - beta;
+    c.V = V;
+    c.A = A;
+    c.Eb = Eb;
+    c.Eu = Eb * a.ph.eta;
+    c.E = Eb * a.ph.eta + 0.5 * a.ph.M1 * ve * ve + 0.5 * a.ph.M2 * se * se;
+    c.l1 = a.ph.M1 * ve;
+    c.l2 = a.ph.M2 * se;
+    c.sum_v = tot[0];
+    c.sum_w = tot[1];
+    c.sum_f2 = tot[2];
+    c.grad2 = tot[3];
+    c.nonfinite = !(isfinite(V) && isfinite(A) && isfinite(Eb)) ? 1 : 0;
+  }
+}
+
+__global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->alpha = alpha;
+    sc->beta = beta;
+    sc->alpha0 = alpha0;
+    sc->beta0 = beta0;
+  }
+}
+
+void launch_finalize(const double* xpart, int nxb, const double* zpart, int nzb, Scal* sc, const Geo& g,
+                     const Phys& ph, int mode, hipStream_t s) {
+  FinArgs a;
+  a.xpart = xpart;
+  a.zpart = zpart;
+  a.sc = sc;
+  a.nxb = nxb;
+  a.nzb = nzb;
+  a.mode = mode;
+  a.pad = 0;
+  a.g = g;
+  a.ph = ph;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, a);
+}
+
+void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_scal, dim3(1), dim3(64), 0, s, sc, alpha, beta, alpha0, beta0);
+}
+
+// ---------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Solver final : SolverBase {
+  Geo g{};
+  Phys ph{};
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  const SizeOps<T>* ox = nullptr;
+  const SizeOps<T>* oz = nullptr;
+  DBuf<cpx<T>> uh[2], pre, acc, P, Q, Tt, W, S;
+  DBuf<T> v, f;
+  DBuf<double> xpart, zpart, phid;
+  DBuf<Scal> scal;
+  DBuf<cpx<T>> twx, twz;
+  Scal* hs = nullptr;   // pinned host copy
+  int nxb = 0, nzb = 0;
+  long long npairs = 0;
+  int mmax = 0;
+  std::map<std::pair<int, double>, std::unique_ptr<DBuf<Coef>>> luts;
+
+  // pipeline state
+  int cur = 0;
+  bool prepped = false;     // P, Q hold zprep of uh[cur]
+  bool prefixed = false;    // prefix of uh[cur] done (v, f, Tt, scal valid)
+  bool prefixed_f = false;  // ... including W (Euler)
+  // driver state
+  bool started = false;
+  double time_step = 0, time_current = 0;
+  int steps = 0;
+  double alpha = 0, beta = 0, alpha0 = 0, beta0 = 0;
+
+  // profiling
+  bool prof = false;
+  struct Rec { const char* name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> evpool;
+  std::map<std::string, std::pair<int, double>> ktimes;
+
+  Solver(const vb_params& pp, hipStream_t shared) {
+    p = pp;
+    g = make_geo(p.nx, p.nz, p.len * (2.0 * 3.1415926535897932384626433832795));
+    ph.eps = p.eps;
+    ph.eta = p.eta;
+    ph.ga = p.ga;
+    ph.kappa = p.kappa;
+    ph.D = p.D;
+    ph.M1 = p.M1;
+    ph.M2 = p.M2;
+    ox = get_ops<T>(p.nx);
+    oz = get_ops<T>(p.nz);
+    if (shared) {
+      st = shared;
+    } else {
+      VB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      own_stream = true;
+    }
+    const size_t nc = (size_t)g.nc, nr = (size_t)g.nr;
+    for (auto* b : {&uh[0], &uh[1], &pre, &acc, &P, &Q, &Tt, &S}) b->alloc(nc);
+    if (p.scheme == VB_EULER || p.shape_adjust) W.alloc(nc);
+    v.alloc(nr);
+    f.alloc(nr);
+    phid.alloc(nr);
+    npairs = (long long)g.nz * g.ny / 2;
+    nxb = int((npairs + ox->xlanes - 1) / ox->xlanes);
+    nzb = int(((long long)g.ny * g.nh + oz->ytile - 1) / oz->ytile);
+    xpart.alloc((size_t)nxb * 3);
+    zpart.alloc((size_t)nzb);
+    scal.alloc(1);
+    VB_HIP(hipHostMalloc((void**)&hs, sizeof(Scal), hipHostMallocDefault));
+    memset(hs, 0, sizeof(Scal));
+    VB_HIP(hipMemsetAsync(scal.p, 0, sizeof(Scal), st));
+    upload_twiddles(twx, p.nx);
+    upload_twiddles(twz, p.nz);
+    mmax = mmax_of(p.nx, p.ny, p.nz);
+    time_step = p.time_step;
+    alpha = p.alpha;
+    beta = p.beta;
+    alpha0 = alpha;
+    beta0 = beta;
+    set_targets_device();
+    VB_HIP(hipStreamSynchronize(st));
+  }
+
+  ~Solver() override {
+    if (st) (void)hipStreamSynchronize(st);
+    flush_prof();
+    for (auto e : evpool) (void)hipEventDestroy(e);
+    if (hs) (void)hipHostFree(hs);
+    if (own_stream && st) (void)hipStreamDestroy(st);
+  }
+
+  void upload_twiddles(DBuf<cpx<T>>& b, int n) {
+    std::vector<double> t;
+    build_twiddles(t, n);
+    std::vector<cpx<T>> h((size_t)n);
+    for (int m = 0; m < n; ++m) {
+      h[(size_t)m].x = T(t[2 * (size_t)m]);
+      h[(size_t)m].y = T(t[2 * (size_t)m + 1]);
+    }
+    b.alloc((size_t)n);
+    VB_HIP(hipMemcpy(b.p, h.data(), sizeof(cpx<T>) * (size_t)n, hipMemcpyHostToDevice));
+  }
+
+  const Coef* lut(int sch, double h) {
+    auto key = std::make_pair(sch, h);
+    auto it = luts.find(key);
+    if (it != luts.end()) return it->second->p;
+    if (luts.size() > 64) {
+      VB_HIP(hipStreamSynchronize(st));
+      luts.clear();
+    }
+    std::vector<Coef> tab;
+    build_lut(tab, mmax, sch, p.phi_lut_mode, h, g.kfac2, p.eps, p.kappa, p.D);
+    auto b = std::make_unique<DBuf<Coef>>();
+    b->alloc(tab.size());
+    VB_HIP(hipMemcpy(b->p, tab.data(), sizeof(Coef) * tab.size(), hipMemcpyHostToDevice));
+    const Coef* r = b->p;
+    luts.emplace(key, std::move(b));
+    return r;
+  }
+
+  // ---- profiling ---------------------------------------------------------
+  hipEvent_t ev() {
+    if (!evpool.empty()) {
+      hipEvent_t e = evpool.back();
+      evpool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    VB_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void flush_prof() {
+    if (pending.empty()) return;
+    (void)hipStreamSynchronize(st);
+    for (auto& r : pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        auto& k = ktimes[r.name];
+        k.first += 1;
+        k.second += ms;
+      }
+      evpool.push_back(r.a);
+      evpool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  template <class F>
+  void L(const char* name, F&& fn) {
+    if (!prof) {
+      fn();
+      return;
+    }
+    Rec r{name, ev(), ev()};
+    VB_HIP(hipEventRecord(r.a, st));
+    fn();
+    VB_HIP(hipEventRecord(r.b, st));
+    pending.push_back(r);
+    if (pending.size() > 2048) flush_prof();
+  }
+  void profile(bool on) override {
+    flush_prof();
+    if (on) ktimes.clear();
+    prof = on;
+  }
+  int kernel_times(char* names, int name_len, int* counts, double* ms, int cap) override {
+    flush_prof();
+    int i = 0;
+    for (auto& kv : ktimes) {
+      if (i >= cap) break;
+      if (names && name_len > 0) {
+        strncpy(names + (size_t)i * name_len, kv.first.c_str(), (size_t)name_len - 1);
+        names[(size_t)i * name_len + name_len - 1] = 0;
+      }
+      if (counts) counts[i] = kv.second.first;
+      if (ms) ms[i] = kv.second.second;
+      ++i;
+    }
+    return (int)ktimes.size();
+  }
+  hipStream_t stream() override { return st; }
+
+  // ---- pipeline pieces ------------------------------------------------------
+  void check_launch() { VB_HIP(hipGetLastError()); }
+
+  void prep(const cpx<T>* vh) {
+    L("zprep", [&] { oz->zprep(vh, P.p, Q.p, zpart.p, g, twz.p, st); });
+    check_launch();
+    prepped = true;
+  }
+
+  void spectral_lap(cpx<T>* b) {
+    L("y_fwd", [&] { ox->ypass(b, nullptr, 1, -1, g, twx.p, st); });
+    L("z_lap", [&] { oz->zlap(b, g, twz.p, st); });
+    L("y_inv", [&] { ox->ypass(b, nullptr, 1, +1, g, twx.p, st); });
+  }
+
+  void prefix(int mode, bool withf) {
+    L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, st); });
+    XaArgs<T> a;
+    a.P = P.p;
+    a.Q = Q.p;
+    a.v = v.p;
+    a.f = f.p;
+    a.Tt = Tt.p;
+    a.W = W.p;
+    a.part = xpart.p;
+    a.tw = twx.p;
+    a.npairs = npairs;
+    a.eps = p.eps;
+    a.kappa = p.kappa;
+    a.inv_n = 1.0 / double(g.nr);
+    L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf, st); });
+    spectral_lap(Tt.p);
+    if (withf) spectral_lap(W.p);
+    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, nzb, scal.p, g, ph, mode, st); });
+    check_launch();
+    prepped = false;
+    prefixed = true;
+    prefixed_f = withf;
+  }
+
+  void refinalize(int mode) {
+    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, nzb, scal.p, g, ph, mode, st); });
+    check_launch();
+  }
+
+  void suffix(int sch, int stg, const Coef* lt, int out) {
+    XbArgs<T> b;
+    b.Tt = Tt.p;
+    b.v = v.p;
+    b.f = f.p;
+    b.sc = scal.p;
+    b.tw = twx.p;
+    b.npairs = npairs;
+    b.eps = p.eps;
+    b.kappa = p.kappa;
+    b.D = p.D;
+    b.eta = p.eta;
+    b.inv_n = 1.0 / double(g.nr);
+    L("xb", [&] { ox->xb(b, st); });
+    L("y_fwd", [&] { ox->ypass(Tt.p, nullptr, 1, -1, g, twx.p, st); });
+    ZsArgs<T> z;
+    z.U = Tt.p;
+    z.uh = uh[cur].p;
+    z.pre = pre.p;
+    z.acc = acc.p;
+    z.uout = uh[out].p;
+    z.P = P.p;
+    z.Q = Q.p;
+    z.part = zpart.p;
+    z.lut = lt;
+    z.tw = twz.p;
+    z.g = g;
+    L("zstage", [&] { oz->zstage(z, sch, stg, st); });
+    check_launch();
+    prefixed = false;
+    prefixed_f = false;
+    prepped = true;   // P, Q hold the next stage field (see below)
+  }
+
+  void ensure_prefixed(bool withf = false) {
+    if (prefixed && (!withf || prefixed_f)) return;
+    if (!prepped) prep(uh[cur].p);
+    prefix(FIN_PLAIN, withf);
+  }
+
+  // one ETDRK4 / ETDRK2 evaluation from uh[cur] into uh[1-cur]; on return P, Q
+  // hold the prepared new state.
+  int eval(double h) {
+    const int sch = p.scheme == VB_ETDRK2 ? 2 : 4;
+    const Coef* lt = lut(sch, h);
+    const int out = 1 - cur;
+    ensure_prefixed();
+    for (int stg = 1; stg <= sch; ++stg) {
+      if (stg > 1) prefix(FIN_PLAIN, false);
+      suffix(sch, stg, lt, out);
+    }
+    // after the last stage P, Q belong to uh[out]
+    return out;
+  }
+
+  // Forward-Euler update of uh[cur] into uh[1-cur] with rhs g0 (shape) or nld1
+  int euler(double t, bool shape) {
+    ensure_prefixed(true);
+    const int out = 1 - cur;
+    XeArgs<T> e;
+    e.W = W.p;
+    e.v = v.p;
+    e.f = f.p;
+    e.out = uh[out].p;
+    e.sc = scal.p;
+    e.tw = twx.p;
+    e.npairs = npairs;
+    e.eps = p.eps;
+    e.eta = p.eta;
+    e.t = t;
+    e.inv_n = 1.0 / double(g.nr);
+    e.shape = shape ? 1 : 0;
+    L("xe", [&] { ox->xe(e, st); });
+    L("y_fwd", [&] { ox->ypass(uh[out].p, nullptr, 1, -1, g, twx.p, st); });
+    L("z_fwd", [&] { oz->zpass(uh[out].p, -1, g, twz.p, st); });
+    check_launch();
+    return out;
+  }
+
+  void read_scal() {
+    VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+  }
+
+  // make candidate `out` current (P, Q already prepared for it by eval)
+  void accept(int out) {
+    cur = out;
+  }
+
+  // ---- drivers ----------------------------------------------------------------
+  void set_targets_device() { launch_fill_scal(scal.p, alpha, beta, alpha0, beta0, st); }
+
+  // Forward-Euler shape relaxation, unconstrained (belastic.cpp:174-283)
+  void shape_adjust() {
+    double tcur = 0.0;
+    int nst = 1;
+    double dt = 1e-7;
+    ensure_prefixed(true);
+    read_scal();
+    double eng0 = hs->Eu, eng1 = eng0, eng = eng0;
+    for (int it = 0; it < 1000000; ++it) {
+      tcur += dt;
+      nst++;
+      if (dt > 1e-4 || int(tcur / dt) % int(1e-4 / dt) == 0) {
+        ensure_prefixed(true);
+        read_scal();
+        eng1 = hs->Eu;
+        if (eng1 < 100 * p.eta || fabs((eng - eng1) / eng) < 0.02) break;
+        eng = eng1;
+      }
+      if (nst % 10 == 0) dt *= 2;
+      for (;;) {
+        const int out = euler(-dt * 3, true);
+        prep(uh[out].p);
+        const int keep = cur;
+        cur = out;
+        prefix(FIN_PLAIN, true);
+        read_scal();
+        eng1 = hs->Eu;
+        if (eng1 < eng0) {
+          eng0 = eng1;
+          break;
+        }
+        cur = keep;   // restore (belastic.cpp:249-250)
+        prefixed = prefixed_f = prepped = false;
+        dt /= 2;
+        if (dt < 1e-13) break;
+      }
+      if (dt < 1e-13) break;
+      if (hs->nonfinite) break;
+    }
+  }
+
+  void start() {
+    started = true;
+    if (p.shape_adjust && time_current == 0.0) shape_adjust();
+    ensure_prefixed(p.scheme == VB_EULER);
+    if (beta <= 0.0) {   // belastic.cpp:588-591
+      read_scal();
+      alpha = hs->V;
+      beta = hs->A;
+    }
+    alpha0 = alpha;   // belastic.cpp:592-593
+    beta0 = beta;
+    set_targets_device();
+    refinalize(FIN_PLAIN);
+    if (p.fixed_time_step > 0) time_step = std::min(p.fixed_time_step, time_step);   // belastic.cpp:599-601
+  }
+
+  void sync_targets_from_device() {
+    read_scal();
+    alpha = hs->alpha;
+    beta = hs->beta;
+  }
+
+  void step(int n, vb_step_report* rep) override {
+    vb_step_report r{};
+    r.status = VB_OK;
+    r.energy = NAN;
+    if (!started) start();
+    const bool adaptive = p.fixed_time_step <= 0;
+    const bool drift = !adaptive && (p.M1 != 0 || p.M2 != 0) && p.penalty_drift;
+    for (int it = 0; it < n; ++it) {
+      double eng0 = NAN;
+      if (p.scheme == VB_EULER) {
+        // run_static loop body (belastic.cpp:367-437)
+        ensure_prefixed(true);
+        if (adaptive) {
+          read_scal();
+          eng0 = hs->E;
+          if (steps % 4 == 0) time_step = time_step * 1.4;
+          for (;;) {
+            if (time_step < 1e-23) break;
+            const int out = euler(-time_step * p.ga, false);
+            r.evaluations++;
+            prep(uh[out].p);
+            const int keep = cur;
+            cur = out;
+            prefix(FIN_PLAIN, true);
+            read_scal();
+            if (hs->E > eng0) {
+              cur = keep;
+              prefixed = prefixed_f = prepped = false;
+              time_step /= 2;
+              r.rejections++;
+              ensure_prefixed(true);
+              continue;
+            }
+            r.energy = hs->E;
+            break;
+          }
+        } else {
+          const int out = euler(-time_step * p.ga, false);
+          r.evaluations++;
+          cur = out;
+          prepped = prefixed = prefixed_f = false;
+        }
+        if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
+        if (time_current > p.tmax) { r.status = VB_FINISHED; break; }
+      } else if (adaptive) {
+        // belastic.cpp:639-669
+        ensure_prefixed();
+        read_scal();
+        eng0 = hs->E;
+        if (steps % p.dt_double_every == 0) {
+          if (time_step * 1.4 < p.dt_cap) time_step = time_step * 2;
+          else time_step = p.dt_cap;
+        }
+        for (;;) {
+          if (time_step < 1e-23) break;
+          const int out = eval(time_step);
+          r.evaluations++;
+          const int keep = cur;
+          cur = out;
+          prefix(FIN_PLAIN, false);
+          read_scal();
+          if (hs->E > eng0) {
+            cur = keep;
+            prefixed = prefixed_f = prepped = false;
+            time_step /= 2;
+            r.rejections++;
+            ensure_prefixed();
+            continue;
+          }
+          r.energy = hs->E;
+          break;
+        }
+      } else if (drift) {
+        // belastic.cpp:670-691
+        ensure_prefixed();
+        int out = eval(time_step);
+        int keep = cur;
+        cur = out;
+        prefix(FIN_DRIFT, false);
+        cur = keep;
+        prepped = prefixed = prefixed_f = false;
+        ensure_prefixed();
+        out = eval(time_step);
+        cur = out;
+        prefix(FIN_DRIFT, false);
+        r.evaluations += 2;
+      } else {
+        ensure_prefixed();
+        const int out = eval(time_step);
+        cur = out;
+        r.evaluations++;
+      }
+      // exits (belastic.cpp:713-740)
+      if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
+      if (adaptive && p.scheme != VB_EULER && eng0 < 1e-3) { r.status = VB_FINISHED; break; }
+      if (adaptive) {
+        if (hs->nonfinite) { r.status = VB_ERR_NONFINITE; break; }
+      }
+      time_current += time_step;
+      steps++;
+      r.steps++;
+    }
+    if (drift) sync_targets_from_device();
+    if (r.status == VB_OK && !adaptive) {
+      // the blow-up test of the driver (belastic.cpp:733-740), on the integrals
+      ensure_prefixed(p.scheme == VB_EULER);
+      read_scal();
+      if (hs->nonfinite) r.status = VB_ERR_NONFINITE;
+    }
+    r.time_step = time_step;
+    r.time_current = time_current;
+    if (rep) *rep = r;
+  }
+
+  void query(double* E, double* V, double* A, double* Eb) override {
+    ensure_prefixed();
+    read_scal();
+    if (E) *E = hs->E;
+    if (V) *V = hs->V;
+    if (A) *A = hs->A;
+    if (Eb) *Eb = hs->Eb;
+  }
+
+  void state(double* a, double* b, double* dt, double* t) override {
+    if (started && p.penalty_drift) sync_targets_from_device();
+    if (a) *a = alpha;
+    if (b) *b = beta;
+    if (dt) *dt = time_step;
+    if (t) *t = time_current;
+  }
+
+  void set_state(double a, double b, double dt, double t, double a0, double b0) override {
+    alpha = a;
+    beta = b;
+    alpha0 = a0;
+    beta0 = b0;
+    time_step = dt;
+    time_current = t;
+    set_targets_device();
+    if (prefixed) refinalize(FIN_PLAIN);
+    started = t > 0.0 || started;
+  }
+
+  void get_phi(double* host) override {
+    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<T>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));
+    L("z_inv", [&] { oz->zpass(S.p, +1, g, twz.p, st); });
+    L("y_inv", [&] { ox->ypass(S.p, nullptr, 1, +1, g, twx.p, st); });
+    L("x_c2r", [&] { ox->xc2r_d(S.p, phid.p, npairs, twx.p, st); });
+    check_launch();
+    VB_HIP(hipMemcpyAsync(host, phid.p, sizeof(double) * (size_t)g.nr, hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+  }
+
+  void set_phi(const double* host) override {
+    VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
+    L("x_r2c", [&] { ox->xr2c_d(phid.p, uh[cur].p, npairs, twx.p, 1.0 / double(g.nr), st); });
+    L("y_fwd", [&] { ox->ypass(uh[cur].p, nullptr, 1, -1, g, twx.p, st); });
+    L("z_fwd", [&] { oz->zpass(uh[cur].p, -1, g, twz.p, st); });
+    check_launch();
+    VB_HIP(hipStreamSynchronize(st));
+    prepped = prefixed = prefixed_f = false;
+  }
+
+  void evaluate(double h) override {
+    const int out = eval(h);
+    cur = out;
+  }
+};
+
+std::unique_ptr<SolverBase> make_solver(const vb_params& p, hipStream_t shared) {
+  if (p.precision == VB_FP32_MIXED) return std::make_unique<Solver<float>>(p, shared);
+  return std::make_unique<Solver<double>>(p, shared);
+}
+
+size_t solver_footprint(const vb_params& p) {
+  const size_t nh = (size_t)p.nx / 2 + 1;
+  const size_t nc = nh * (size_t)p.nx * (size_t)p.nz, nr = (size_t)p.nx * p.nx * p.nz;
+  const size_t cs = p.precision == VB_FP32_MIXED ? 8 : 16, rs = cs / 2;
+  size_t ncb = 8 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
+  const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
+  return ncb * nc * cs + 2 * nr * rs + nr * 8 + 2 * mm * sizeof(Coef) + (size_t)4 << 20;
+}
+
+}  // namespace vb

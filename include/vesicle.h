@@ -1,0 +1,194 @@
+/*
+ * vesicle.h -- C ABI of the MI355X-native pseudo-spectral vesicle solver
+ * (libvesicle.so).  It is a drop-in for cudaBubble's solver path: the
+ * Allen-Cahn phase-field elastic-bending vesicle model on a periodic
+ * [-pi, pi]^3 grid, with volume and area penalties, spectral derivatives and
+ * Forward-Euler / ETDRK2 / ETDRK4 time stepping, on one AMD Instinct GPU.
+ *
+ * The reference has no C ABI: the path sits behind three C++ classes,
+ * belastic (model and drivers), phasefield (one field with cached derived
+ * quantities) and Calc (FFT plans and kernels).  Each entry point below names
+ * the reference routine it replaces.  Everything is plain C: host pointers are
+ * caller-owned copies, device pointers (the vb_calc_* level) are plain
+ * addresses of device memory, and every call returns a vb_status code.
+ *
+ * Layouts follow the reference (Calc.h:4, Calc.cu:57-58, tool.cpp:28-42):
+ * real fields are [nz][ny][nx] with x fastest; half spectra are
+ * [nz][ny][nx/2+1] complex values, interleaved (re, im).
+ */
+#ifndef VESICLE_H
+#define VESICLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vb_status {
+  VB_OK = 0,
+  VB_ERR_PARAM = 1,        /* invalid parameter or unsupported grid size        */
+  VB_ERR_ODD_NX = 2,       /* nx must be even (Calc.cu:52-55)                   */
+  VB_ERR_OOM = 3,          /* device allocation failed                          */
+  VB_ERR_HIP = 4,          /* a HIP runtime call failed                          */
+  VB_ERR_NONFINITE = 5,    /* the field blew up (belastic.cpp:733-740)          */
+  VB_ERR_DT_UNDERFLOW = 6, /* time step < 1e-23 (belastic.cpp:713-718)          */
+  VB_ERR_IO = 7,           /* file open / read / write failed                    */
+  VB_ERR_NO_DEVICE = 8,    /* no usable GPU                                      */
+  VB_FINISHED = 9          /* a normal driver exit: E < 1e-3 (belastic.cpp:727-732),
+                              shape-adjust convergence or t > tmax (belastic.cpp:407) */
+} vb_status;
+
+enum { VB_EULER = 1, VB_ETDRK2 = 2, VB_ETDRK4 = 4 };
+enum { VB_FP64 = 0, VB_FP32_MIXED = 1 };
+enum { VB_LUT_REFERENCE = 0, VB_LUT_ACCURATE = 1 };
+
+/* Solver parameters.  Field names follow fluidbase/belastic members.
+ * vb_params_default() fills the reference defaults. */
+typedef struct vb_params {
+  int nx, ny, nz;          /* grid; ny == nx, nx even (Calc.cu:45-55)           */
+  int scheme;              /* VB_ETDRK4 (run_static_fEIF), VB_ETDRK2, VB_EULER  */
+  double len;              /* box length in units of 2 pi (fluidbase.cpp:52-57)*/
+  double eps;              /* interface width; <= 0: from epssize             */
+  int epssize;             /* ini code for eps (fluidbase.cpp:68-108)           */
+  int precision;           /* VB_FP64 or VB_FP32_MIXED                          */
+  double eta, ga;          /* bending rigidity, Euler mobility gamma            */
+  double M1, M2;           /* volume / area penalty weights                     */
+  double alpha, beta;      /* targets; beta <= 0: taken from phi0 at t = 0      */
+  double kappa, D;         /* 2 and 1.5 (Calc.cu:581-582)                       */
+  double time_step;        /* initial dt (fluidbase ini field time_step)        */
+  double fixed_time_step;  /* > 0: fixed dt = min(fixed, time_step)            */
+  double dt_cap;           /* adaptive dt cap, 1e-3 (belastic.cpp:645)          */
+  int dt_double_every;     /* adaptive doubling period, 4 (belastic.cpp:644)    */
+  int penalty_drift;       /* 1: fixed-dt alpha/beta drift step (belastic.cpp:670-691) */
+  int phi_lut_mode;        /* VB_LUT_REFERENCE or VB_LUT_ACCURATE               */
+  int shape_adjust;        /* b_shape: Euler pre-relaxation at t = 0            */
+  double tmax;             /* Euler driver stop time (belastic.cpp:407)         */
+  int device;              /* HIP device ordinal                                */
+  int pad;
+} vb_params;
+
+/* Counters of one vb_step call. */
+typedef struct vb_step_report {
+  int steps;               /* accepted driver iterations                        */
+  int evaluations;         /* stepper evaluations (ETDRK4 calls)                */
+  int rejections;          /* adaptive rejections                               */
+  int status;              /* vb_status of the driver exit condition, VB_OK if none */
+  double time_step;        /* dt after the call                                 */
+  double time_current;     /* simulated time after the call                     */
+  double energy;           /* E of the last accepted state (adaptive) or NaN    */
+} vb_step_report;
+
+typedef struct vb_solver vb_solver;
+typedef struct vb_calc vb_calc;
+
+/* ---- parameters --------------------------------------------------------- */
+void vb_params_default(vb_params* p);
+/* Reads line 1 of an ini<i>.dat file: 34 whitespace fields in the order of
+ * fluidbase::fluidbase (fluidbase.cpp:33-35) and applies its defaults and the
+ * eps table (fluidbase.cpp:52-148). */
+int vb_params_from_ini(const char* path, vb_params* out);
+/* eps for an ini epssize code (fluidbase.cpp:68-108); L is the box length. */
+double vb_eps_from_code(int epssize, int nx, double L);
+
+/* ---- solver (replaces belastic + phasefield) ------------------------------ */
+/* belastic::belastic + phasefield ctor (belastic.cpp:15-23, phasefield.cpp:14-58):
+ * phi0_host is [nz][ny][nx] doubles, copied. */
+int vb_create(const vb_params* p, const double* phi0_host, vb_solver** out);
+void vb_destroy(vb_solver* s);
+/* n iterations of the driver loop of belastic::run_static_fEIF
+ * (belastic.cpp:635-746), or of run_static (belastic.cpp:367-437) for
+ * VB_EULER.  The first call performs the start of the driver
+ * (belastic.cpp:558-618): shape adjustment, alpha/beta auto-init, fixed dt. */
+int vb_step(vb_solver* s, int n_steps, vb_step_report* rep);
+/* belastic::getstaticenergy + phasefield::vol/surface/elastic_eng
+ * (belastic.cpp:68-86, phasefield.cpp:283-325) of the current field. */
+int vb_query(vb_solver* s, double* E, double* V, double* A, double* Eb);
+/* current targets alpha, beta, dt and time */
+int vb_state(vb_solver* s, double* alpha, double* beta, double* time_step, double* time_current);
+int vb_get_phi(vb_solver* s, double* host_out);        /* cudaMemcpy D2H, belastic.cpp:163 */
+int vb_set_phi(vb_solver* s, const double* host_in);   /* phasefield ctor H2D copy        */
+/* one stepper evaluation without the driver: ETD4RK / ETD2RK
+ * (belastic.cpp:519-552, 491-517) with step h */
+int vb_evaluate(vb_solver* s, double h);
+/* checkpoint: <dir>/<nx>x<ny>x<nz>.info (7 doubles, fluidbase.cpp:2776-2795)
+ * and <dir>/d<nx>x<ny>x<nz>.out (raw doubles, fluidbase.cpp:224-265) */
+int vb_save(vb_solver* s, const char* dir);
+int vb_load(vb_solver* s, const char* dir);
+/* device stream the solver launches on (a hipStream_t) */
+void* vb_stream(vb_solver* s);
+/* per-kernel timing: when on, every launch is bracketed by events on the
+ * solver stream; vb_kernel_times returns up to cap entries of
+ * (name, launches, total ms). */
+int vb_profile(vb_solver* s, int on);
+int vb_kernel_times(vb_solver* s, char* names, int name_len, int* counts, double* ms, int cap);
+/* bytes of device memory the solver holds (admission sizing) */
+size_t vb_footprint(const vb_params* p);
+const char* vb_strerror(int status);
+
+/* ---- Calc level (replaces class Calc, Calc.h:8-119) ------------------------
+ * Device pointers: real fields double* / float* [nz][ny][nx]; half spectra
+ * interleaved complex [nz][ny][nx/2+1].  Precision VB_FP64 unless noted. */
+int vb_calc_create(int nx, int nz, double lenx, int precision, vb_calc** out);  /* Calc::Calc, Calc.cu:43-64 */
+void vb_calc_destroy(vb_calc* c);                                             /* Calc::~Calc, Calc.cu:66-75 */
+int vb_calc_r2c(vb_calc* c, const void* u, void* s);              /* r2c_3d_f, Calc.cu:88-96 (FP64/FP32) */
+int vb_calc_c2r(vb_calc* c, const void* s, void* u);              /* c2r_3d_b, Calc.cu:99-104 (FP64/FP32); s is preserved */
+int vb_calc_laplace(vb_calc* c, void* s);                         /* laplace, Calc.cu:127-137 */
+int vb_calc_lap_spec(vb_calc* c, const void* fd, void* out);      /* lap(fd,out), Calc.cu:139-144 */
+int vb_calc_lap_real(vb_calc* c, const void* d, void* out);       /* lap(d,out), Calc.cu:146-151 */
+int vb_calc_fftgradient(vb_calc* c, const void* fd, void* dx, void* dy, void* dz); /* Calc.cu:344-356 */
+int vb_calc_f(vb_calc* c, void* f, const void* lap, const void* d, double eps);    /* Calc.cu:165-169 */
+int vb_calc_g0(vb_calc* c, void* g0, const void* d, const void* f, double eps);    /* Calc.cu:184-188 */
+int vb_calc_intf(vb_calc* c, const void* u, double addition, double multiply, double* host_val); /* Calc.cu:255-263 */
+int vb_calc_nld(vb_calc* c, void* nldp, const void* d, const void* f, void* tmp1,
+                double eps, double kappa, double D);                              /* Calc.cu:286-297 */
+int vb_calc_s_term(vb_calc* c, void* s, const void* d, const void* dx, const void* dy,
+                   const void* dz, double eps);                                   /* Calc.cu:370-374 */
+int vb_calc_multi(vb_calc* c, const void* a, const void* b, void* out);          /* Calc.cu:387-391 */
+int vb_calc_plus(vb_calc* c, void* u, const void* nld1, double t);               /* Calc.cu:404-408 */
+int vb_calc_nld1(vb_calc* c, void* nld, const void* f, const void* g, double l1, double l2, double eta); /* Calc.cu:422-426 */
+int vb_calc_nld2(vb_calc* c, void* nld, const void* f, const void* g, double l1, double l2, double eta); /* Calc.cu:440-444 */
+/* L^ table (Calc.cu:579-606), written to device memory [nz][ny][nx/2+1] doubles */
+int vb_calc_init_fEIF(vb_calc* c, void* Het, double eps);
+/* ETD combines (Calc.cu:446-577).  The coefficient functions of L^ come from
+ * host-built FP64 tables indexed by |k|^2 (lut_mode VB_LUT_REFERENCE keeps the
+ * reference expressions), so no device exp enters the result. */
+int vb_calc_etd1(vb_calc* c, void* a, const void* fd, const void* nld, double eps, double h);
+int vb_calc_etd2(vb_calc* c, void* a, const void* fd, const void* nnld, const void* nld, double eps, double h);
+int vb_calc_etd41(vb_calc* c, void* d_real, const void* fd, const void* nld, double eps, double h);
+int vb_calc_etd42(vb_calc* c, void* b, const void* fd, const void* nlda, double eps, double h);
+int vb_calc_etd43(vb_calc* c, void* cc, const void* nld, const void* nldb, double eps, double h);
+int vb_calc_etd44(vb_calc* c, void* d_real, const void* fd, const void* nlda, const void* nldb,
+                  const void* nldc, double eps, double h);
+int vb_calc_sync(vb_calc* c);
+
+/* ---- initial shapes (fluidbase init_* generators, seeded) ----------------- */
+/* phi = tanh(d / (sqrt2 eps)) on the reference grid x_i = i L/n - L/2
+ * (fluidbase.cpp:152-161).  kind: 0 sphere, 1 ellipsoid, 2 two ellipsoids,
+ * 3 torus, 4 biconcave disk.  Shape parameters are drawn from splitmix64(seed)
+ * in the ranges of the benchmark configurations; noise > 0 adds N(0, noise). */
+int vb_shape(int kind, unsigned long long seed, int nx, int nz, double len, double eps,
+             double noise, double* phi_out, double* shape_params /* optional [8] */);
+
+/* ---- batched solves (many independent vesicles, one launch per pass) ------- */
+typedef struct vb_batch vb_batch;
+int vb_batch_create(int n, const vb_params* params, const double* const* phi0, vb_batch** out);
+int vb_batch_step(vb_batch* b, int n_steps);
+int vb_batch_query(vb_batch* b, int i, double* E, double* V, double* A, double* Eb);
+int vb_batch_get_phi(vb_batch* b, int i, double* host_out);
+void vb_batch_destroy(vb_batch* b);
+
+/* The ETD coefficient table the solver uses for step h (host only, no GPU):
+ * out[m][8] for m = kx^2 + ky'^2 + kz'^2 in [0, mmax] with folded indices.
+ * ETDRK4 records {e1, (e1-1)/H, e2, A s, 2 B s, C s, 0, H} (Calc.cu:495-567,
+ * s = 1/(h^2 H^3)); ETDRK2 records {e, (e-1)/H, c2, 0, 0, 0, 0, H}
+ * (Calc.cu:453-475); H = L^ of Calc::init_fEIF (Calc.cu:599-600). */
+int vb_lut(int scheme, int lut_mode, double h, double eps, double len, double kappa, double D, int mmax,
+           double* out);
+
+const char* vb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
