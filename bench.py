@@ -1,0 +1,205 @@
+"""Benchmark: ETDRK4 grid-point-steps/s at 256^3, FP32-mixed vs FP64.
+
+Workload (the precision configuration): 256^3 grid on [-pi, pi]^3, eps = 2h,
+seeded torus (R ~ U[1.2, 1.8], r ~ U[0.4, 0.7]) with a tanh profile plus
+N(0, 1e-3) noise, ETDRK4 with fixed dt = 5e-7, M1 = M2 = 1e4, alpha/beta from
+t = 0.  With fixed dt and penalties on, one driver step of the reference is two
+ETDRK4 evaluations with the alpha/beta drift correction (belastic.cpp:670-691).
+
+A "step" below is one driver step; the unit of work is one ETDRK4 evaluation
+on one grid point, so value = 256^3 * evaluations / second (FP32-mixed: FP32
+FFTs and pointwise stages, FP64 coefficient tables, reductions and penalty
+logic).  The FP64 rate, the FP32-vs-FP64 error after the same steps, the
+per-step latency percentiles, the roofline of the dominant kernel and a CPU
+baseline (the FP64 oracle on the host cores, a bounded sample) are reported
+in the same JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ETDRK4 grid-pt-steps/s at 256³ (FP32-mixed vs FP64) + p50/p99 sweep latency"
+HBM_PEAK = 8.0e12    # MI355X HBM3E, bytes/s
+
+
+def alg_bytes(name, n, prec_bytes):
+    """Algorithmic bytes of one launch of a pipeline kernel (reads + writes of
+    its arrays; coefficient tables and twiddles excluded as cache-resident)."""
+    cs = 2 * prec_bytes
+    C = (n // 2 + 1) * n * n * cs
+    R = n ** 3 * prec_bytes
+    table = {
+        "y_inv2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_fwd": 2 * C, "z_inv": 2 * C,
+        "xa": 3 * C + 2 * R, "xa_f": 4 * C + 2 * R, "xb": 2 * C + 2 * R, "zprep": 3 * C,
+        # stage combines: N spectrum + state/accumulators + next-stage P, Q
+        "zstage": 21 * C / 4.0,
+    }
+    return table.get(name)
+
+
+def config_params(vb, n, precision, lut_mode):
+    eps = 2.0 * (2 * np.pi) / n
+    phi, sp = vb.shape(vb.SHAPE_TORUS, 20260003, n, n, 1.0, eps, noise=1e-3)
+    p = vb.params_default(nx=n, nz=n, eps=eps, M1=1e4, M2=1e4, alpha=0.0, beta=0.0,
+                          time_step=5e-7, fixed_time_step=5e-7, precision=precision,
+                          phi_lut_mode=lut_mode, scheme=vb.ETDRK4)
+    return p, phi, sp
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def run_timed(vb, p, phi, warmup, steps):
+    s = vb.Solver(p, phi)
+    r = s.step(warmup) if warmup else None
+    sync()
+    t0 = time.perf_counter()
+    r = s.step(steps)
+    sync()
+    t1 = time.perf_counter()
+    assert r["status"] == 0, r
+    return s, (t1 - t0), r["evaluations"]
+
+
+def latency(s, nsteps):
+    out = []
+    for _ in range(nsteps):
+        sync()
+        t0 = time.perf_counter()
+        s.step(1)
+        sync()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out
+
+
+def kernel_profile(vb, s, n, prec_bytes, steps=2):
+    s.profile(True)
+    s.step(steps)
+    kt = s.kernel_times()
+    s.profile(False)
+    rows = []
+    for name, (cnt, ms) in kt.items():
+        b = alg_bytes(name, n, prec_bytes)
+        rows.append((ms, name, cnt, b))
+    rows.sort(reverse=True)
+    return rows
+
+
+def cpu_baseline(n, seconds_cap=30.0):
+    """The FP64 oracle (own FFT, OpenMP) on this host: one ETDRK4 evaluation at n^3."""
+    from oracle import pyoracle
+    import cudabubble_amd as vb
+    eps = 2.0 * (2 * np.pi) / n
+    phi, _ = vb.shape(vb.SHAPE_TORUS, 20260003, n, n, 1.0, eps, noise=1e-3)
+    o = pyoracle.Oracle(n, eps=eps, M1=1e4, M2=1e4)
+    o.set_targets(o.vol(phi), o.surface(phi))
+    t0 = time.perf_counter()
+    d = o.etd4rk(phi, 5e-7)
+    dt = time.perf_counter() - t0
+    evals = 1
+    if dt < seconds_cap / 3:
+        t0 = time.perf_counter()
+        d = o.etd4rk(d, 5e-7)
+        dt = time.perf_counter() - t0
+    return {"value": n ** 3 * evals / dt, "unit": "grid-point-steps/s", "cores": pyoracle.lib().vbo_threads(),
+            "kind": "port",
+            "sample": "1 ETDRK4 evaluation (reference dataflow, 36 FFTs) of the same %d^3 torus in FP64 "
+                      "by the CPU oracle, %.1f s" % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--latency-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the solver path runs on a single device and "
+              "independent solves shard trivially; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cudabubble_amd as vb
+    vb.lib()
+    n = args.n
+
+    # FP32-mixed (headline)
+    p32, phi, sp = config_params(vb, n, vb.FP32_MIXED, vb.LUT_ACCURATE)
+    s32, t32, ev32 = run_timed(vb, p32, phi, args.warmup, args.steps)
+    value = n ** 3 * ev32 / t32
+    ms_step = t32 * 1e3 / args.steps
+    lat = latency(s32, args.latency_steps)
+    prof32 = kernel_profile(vb, s32, n, 4)
+
+    out_extra = {}
+    if not args.no_fp64:
+        p64, _, _ = config_params(vb, n, vb.FP64, vb.LUT_ACCURATE)
+        s64, t64, ev64 = run_timed(vb, p64, phi, args.warmup, args.steps)
+        v64 = n ** 3 * ev64 / t64
+        # both solvers have now taken warmup + steps + latency + profile steps? align them
+        done32 = args.warmup + args.steps + args.latency_steps + 2
+        done64 = args.warmup + args.steps
+        s64.step(done32 - done64)
+        phi32, phi64 = s32.get_phi(), s64.get_phi()
+        E32, V32, A32, Eb32 = s32.query()
+        E64, V64, A64, Eb64 = s64.query()
+        rl2 = float(np.linalg.norm(phi32 - phi64) / np.linalg.norm(phi64))
+        prof64 = kernel_profile(vb, s64, n, 8)
+        out_extra["fp64"] = {"value": v64, "ms_per_step": t64 * 1e3 / args.steps,
+                             "dominant_kernel": prof64[0][1]}
+        out_extra["speedup_fp32_vs_fp64"] = value / v64
+        out_extra["error_vs_fp64"] = {"steps": done32, "phi_rel_l2": rl2,
+                                      "E_rel": abs(E32 - E64) / abs(E64), "V_rel": abs(V32 - V64) / abs(V64),
+                                      "A_rel": abs(A32 - A64) / abs(A64), "Eb_rel": abs(Eb32 - Eb64) / abs(Eb64)}
+
+    ms, name, cnt, b = prof32[0]
+    avg = ms / cnt * 1e-3
+    achieved = b / avg if b else None
+    roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
+            "traffic": None, "avg_launch_us": avg * 1e6, "alg_bytes_per_launch": b}
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%d_fp32.json" % (name, n))
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            roof["traffic"] = json.load(f).get("bytes_per_launch")
+    # whole-evaluation roofline against the canonical fused dataflow (35 C + 44 R bytes)
+    C4 = (n // 2 + 1) * n * n * 8
+    R4 = n ** 3 * 4
+    B_alg = 35 * C4 + 44 * R4
+    t_eval = t32 / ev32
+    out_extra["step_roofline"] = {"B_alg_per_evaluation": B_alg, "achieved_GBps": B_alg / t_eval / 1e9,
+                                  "frac": B_alg / t_eval / HBM_PEAK}
+    out_extra["latency_ms_per_step"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                        "n": len(lat)}
+    out_extra["kernels_fp32"] = [{"name": nm, "launches": c, "ms": round(m, 3),
+                                  "GBps": (bb / (m / c * 1e-3) / 1e9) if bb else None}
+                                 for m, nm, c, bb in prof32]
+    cpu = None if args.no_cpu else cpu_baseline(n)
+
+    line = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "%d^3 torus (seed 20260003, noise 1e-3), eps=2h, ETDRK4 fixed dt=5e-7, "
+                                   "M1=M2=1e4 penalty drift (2 evaluations per step), FP32-mixed" % n,
+                       "grid": n, "evaluations_per_step": ev32 / args.steps, "shape": [round(x, 6) for x in sp[:2]],
+                       "lut": "accurate", "precision": "fp32-mixed (f64 tables, reductions, penalty/dt logic)",
+                       "parallelism": "1 GPU"},
+            "roofline": roof, "cpu_baseline": cpu}
+    line.update(out_extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -68,6 +68,14 @@ def lib():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libvesicle.so is not built (run __graft_entry__.build())")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and
+    # loads it by path, so a process that uses both must load torch first; the
+    # library's DT_NEEDED libamdhip64.so.7 then binds to that same runtime
+    # instead of starting a second one that would not see the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     I, D, P, S = C.c_int, C.c_double, C.c_void_p, C.c_size_t
     L.vb_version.restype = C.c_char_p

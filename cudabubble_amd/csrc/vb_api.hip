@@ -97,7 +97,12 @@ const char* vb_strerror(int s) {
     case VB_ERR_NONFINITE: return "the field blew up (non-finite values)";
     case VB_ERR_DT_UNDERFLOW: return "time step < 1e-23";
     case VB_ERR_IO: return "file input/output failed";
-    case VB_ERR_NO_DEVICE: return "no usable GPU";
+    case VB_ERR_NO_DEVICE: {
+      static thread_local std::string m;
+      std::lock_guard<std::mutex> l(vb::g_err_mu);
+      m = "no usable GPU (" + vb::g_err + ")";
+      return m.c_str();
+    }
     case VB_FINISHED: return "driver finished";
     default: return "unknown status";
   }
@@ -207,8 +212,12 @@ int vb_create(const vb_params* p, const double* phi0, vb_solver** out) {
   if (rc != VB_OK) return rc;
   if (!phi0) return VB_ERR_PARAM;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || p->device < 0 || p->device >= ndev)
+  const hipError_t de = hipGetDeviceCount(&ndev);
+  if (de != hipSuccess || ndev <= 0 || p->device < 0 || p->device >= ndev) {
+    vb::set_last_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(de) + ", devices " +
+                       std::to_string(ndev));
     return VB_ERR_NO_DEVICE;
+  }
   return guard([&] {
     VB_HIP(hipSetDevice(p->device));
     auto s = std::make_unique<vb_solver>();

@@ -367,7 +367,12 @@ int vb_calc_create(int nx, int nz, double lenx, int precision, vb_calc** out) {
   if (!vb::size_supported(nx) || !vb::size_supported(nz) || lenx <= 0) return VB_ERR_PARAM;
   if (precision != VB_FP64 && precision != VB_FP32_MIXED) return VB_ERR_PARAM;
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VB_ERR_NO_DEVICE;
+  const hipError_t de = hipGetDeviceCount(&ndev);
+  if (de != hipSuccess || ndev <= 0) {
+    vb::set_last_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(de) + ", devices " +
+                       std::to_string(ndev));
+    return VB_ERR_NO_DEVICE;
+  }
   auto c = std::make_unique<vb_calc>();
   int rc = guard([&] { c->impl = std::make_unique<CalcImpl>(nx, nz, lenx, precision); });
   if (rc == VB_OK) *out = c.release();

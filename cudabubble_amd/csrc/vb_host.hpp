@@ -65,8 +65,8 @@ struct DBuf {
 Geo make_geo(int nx, int nz, double L);
 
 // finalize kernel launcher (vb_solver.hip)
-void launch_finalize(const double* xpart, int nxb, const double* zpart, int nzb, Scal* sc, const Geo& g,
-                     const Phys& ph, int mode, hipStream_t s);
+void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
+                     const Geo& g, const Phys& ph, int mode, hipStream_t s);
 void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s);
 
 }  // namespace vb

@@ -293,10 +293,10 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
 // y kernels: complex transforms along y, in place, batch over (z, kx).
 // blockIdx.y selects one of two arrays.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int S>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_ypass(cpx<T>* b0, cpx<T>* b1, Geo g,
-                                                                   const cpx<T>* __restrict__ tw) {
-  constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
+template <typename T, int N, int S, int TB = tbw<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(cpx<T>* b0, cpx<T>* b1, Geo g,
+                                                          const cpx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   cpx<T>* buf = blockIdx.y ? b1 : b0;
   const int c = threadIdx.x % TB, j = threadIdx.x / TB;
@@ -354,9 +354,9 @@ __device__ __forceinline__ int mindex(const Geo& g, int kx, int ky, int kz) {
   return kx * kx + fy * fy + fz * fz;
 }
 
-template <typename T, int N, int S>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zpass(cpx<T>* buf, Geo g, const cpx<T>* __restrict__ tw) {
-  constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
+template <typename T, int N, int S, int TB = tbw<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zpass(cpx<T>* buf, Geo g, const cpx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
@@ -374,9 +374,9 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zpass(cpx<T>* buf
 }
 
 // forward along z, multiply by -|k|^2 (laplace_kernel, Calc.cu:114-122), inverse
-template <typename T, int N>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zlap(cpx<T>* buf, Geo g, const cpx<T>* __restrict__ tw) {
-  constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
+template <typename T, int N, int TB = tbw<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(cpx<T>* buf, Geo g, const cpx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
@@ -400,29 +400,32 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zlap(cpx<T>* buf,
   }
 }
 
-// Shared tail of zprep / zstage: Parseval partial of the next field nf, then
-// P = z-inverse(nf), Q = z-inverse(-|k|^2 nf).
+// Shared tail of zprep / zstage.  nf is the next stage field in FP64 (the
+// state and the combine stay FP64 in both precisions): its Parseval partial
+// of |grad|^2 and its mean (the k = 0 coefficient, written to dc[0]) are taken
+// in FP64; then P = z-inverse(nf), Q = z-inverse(-|k|^2 nf) in T.
 template <typename T, int N, int TB, class Lds>
-__device__ __forceinline__ void next_field_tail(cpx<T>* nf, int j, const ZIdx& id, const Geo& g, Lds& lds,
+__device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, const ZIdx& id, const Geo& g, Lds& lds,
                                                 const cpx<T>* __restrict__ tw, cpx<T>* P, cpx<T>* Q,
-                                                double* part, double* red) {
+                                                double* part, double* dc, double* red) {
   constexpr int E = Plan<N>::E, TF = N / E;
   const long long plane = (long long)g.ny * g.nh;
   double acc[1] = {0.0};
-  cpx<T> y[E];
+  cpx<T> x[E], y[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
-    if (id.valid) {
-      const double re = double(nf[e].x), im = double(nf[e].y);
-      acc[0] += grad_w(g, id.kx, id.ky, kz) * (re * re + im * im);
-    }
-    y[e] = scale(nf[e], T(-double(mindex(g, id.kx, id.ky, kz)) * g.kfac2));
+    const double re = nf[e].x, im = nf[e].y;
+    if (id.valid) acc[0] += grad_w(g, id.kx, id.ky, kz) * (re * re + im * im);
+    const double m = -double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
+    x[e] = mk<T>(T(re), T(im));
+    y[e] = mk<T>(T(re * m), T(im * m));
   }
-  fft<T, N, +1>(nf, j, tw, lds);
+  if (id.q == 0 && j == 0) dc[0] = nf[0].x;
+  fft<T, N, +1>(x, j, tw, lds);
   if (id.valid) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) P[id.q + (long long)(j + TF * e) * plane] = nf[e];
+    for (int e = 0; e < E; ++e) P[id.q + (long long)(j + TF * e) * plane] = x[e];
   }
   fft<T, N, +1>(y, j, tw, lds);
   if (id.valid) {
@@ -433,8 +436,9 @@ __device__ __forceinline__ void next_field_tail(cpx<T>* nf, int j, const ZIdx& i
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zprep(const cpx<T>* __restrict__ vh, cpx<T>* P, cpx<T>* Q,
-                                                                   double* part, Geo g, const cpx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zprep(const cpx<double>* __restrict__ vh, cpx<T>* P,
+                                                                   cpx<T>* Q, double* part, double* dc, Geo g,
+                                                                   const cpx<T>* __restrict__ tw) {
   constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
   __shared__ cpx<T> sm[N * TB];
   __shared__ double red[(TB * TF) / 64];
@@ -442,22 +446,23 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zprep(const cpx<T
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
   const long long plane = (long long)g.ny * g.nh;
-  cpx<T> x[E];
+  cpx<double> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = vh[id.q + (long long)(j + TF * e) * plane];
-  next_field_tail<T, N, TB>(x, j, id, g, lds, tw, P, Q, part, red);
+  next_field_tail<T, N, TB>(x, j, id, g, lds, tw, P, Q, part, dc, red);
 }
 
 template <typename T>
 struct ZsArgs {
-  const cpx<T>* U;   // x,y-forward of N (normalised)
-  const cpx<T>* uh;  // state spectrum
-  cpx<T>* pre;       // stage precombination
-  cpx<T>* acc;       // final-combine accumulator
-  cpx<T>* uout;      // new state spectrum (last stage)
+  const cpx<T>* U;        // x,y-forward of N (normalised)
+  const cpx<double>* uh;  // state spectrum
+  cpx<double>* pre;       // stage precombination
+  cpx<double>* acc;       // final-combine accumulator
+  cpx<double>* uout;      // new state spectrum (last stage)
   cpx<T>* P;
   cpx<T>* Q;
   double* part;
+  double* dc;
   const Coef* lut;
   const cpx<T>* tw;
   Geo g;
@@ -479,6 +484,7 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zstage(ZsArgs<T> 
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
   fft<T, N, -1>(x, j, a.tw, lds);   // x = N^ of this stage
+  cpx<double> nf[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
@@ -488,48 +494,54 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zstage(ZsArgs<T> 
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
       const double e1 = cf.c[0], q = cf.c[1];
-      const cpx<T> u = a.uh[i];
-      fr = e1 * double(u.x) + q * nr;                 // etd41: a^
-      fi = e1 * double(u.y) + q * ni;
-      if (id.valid) a.pre[i] = mk<T>(T(e1 * fr - q * nr), T(e1 * fi - q * ni));
+      const cpx<double> u = a.uh[i];
+      fr = e1 * u.x + q * nr;                         // etd41: a^
+      fi = e1 * u.y + q * ni;
+      if (id.valid) a.pre[i] = mk<double>(e1 * fr - q * nr, e1 * fi - q * ni);
     } else if constexpr (SCH == 4 && STG == 2) {
       const double e1 = cf.c[0], q = cf.c[1], e2 = cf.c[2], fa = cf.c[3], fb = cf.c[4];
-      const cpx<T> u = a.uh[i];
-      fr = e1 * double(u.x) + q * nr;                 // etd42: b^
-      fi = e1 * double(u.y) + q * ni;
-      if (id.valid)
-        a.acc[i] = mk<T>(T(e2 * double(u.x) + fa * nr + fb * nr), T(e2 * double(u.y) + fa * ni + fb * ni));
+      const cpx<double> u = a.uh[i];
+      fr = e1 * u.x + q * nr;                         // etd42: b^
+      fi = e1 * u.y + q * ni;
+      if (id.valid) a.acc[i] = mk<double>(e2 * u.x + fa * nr + fb * nr, e2 * u.y + fa * ni + fb * ni);
     } else if constexpr (SCH == 4 && STG == 3) {
       const double q = cf.c[1], fb = cf.c[4];
-      const cpx<T> p = a.pre[i];
-      fr = double(p.x) + 2.0 * q * nr;                // etd43: c^
-      fi = double(p.y) + 2.0 * q * ni;
+      const cpx<double> p = a.pre[i];
+      fr = p.x + 2.0 * q * nr;                        // etd43: c^
+      fi = p.y + 2.0 * q * ni;
       if (id.valid) {
-        const cpx<T> ac = a.acc[i];
-        a.acc[i] = mk<T>(T(double(ac.x) + fb * nr), T(double(ac.y) + fb * ni));
+        const cpx<double> ac = a.acc[i];
+        a.acc[i] = mk<double>(ac.x + fb * nr, ac.y + fb * ni);
       }
     } else if constexpr (SCH == 4 && STG == 4) {
       const double fc = cf.c[5];
-      const cpx<T> ac = a.acc[i];
-      fr = double(ac.x) + fc * nr;                    // etd44: u^+
-      fi = double(ac.y) + fc * ni;
-      if (id.valid) a.uout[i] = mk<T>(T(fr), T(fi));
+      const cpx<double> ac = a.acc[i];
+      fr = ac.x + fc * nr;                            // etd44: u^+
+      fi = ac.y + fc * ni;
+      if (id.valid) a.uout[i] = mk<double>(fr, fi);
     } else if constexpr (SCH == 2 && STG == 1) {
       const double ez = cf.c[0], q1 = cf.c[1], c2 = cf.c[2];
-      const cpx<T> u = a.uh[i];
-      fr = ez * double(u.x) + q1 * nr;                // etd1: a^
-      fi = ez * double(u.y) + q1 * ni;
-      if (id.valid) a.pre[i] = mk<T>(T(fr - c2 * nr), T(fi - c2 * ni));
+      const cpx<double> u = a.uh[i];
+      fr = ez * u.x + q1 * nr;                        // etd1: a^
+      fi = ez * u.y + q1 * ni;
+      if (id.valid) a.pre[i] = mk<double>(fr - c2 * nr, fi - c2 * ni);
     } else {                                          // SCH == 2 && STG == 2
       const double c2 = cf.c[2];
-      const cpx<T> p = a.pre[i];
-      fr = double(p.x) + c2 * nr;                     // etd2: u^+
-      fi = double(p.y) + c2 * ni;
-      if (id.valid) a.uout[i] = mk<T>(T(fr), T(fi));
+      const cpx<double> p = a.pre[i];
+      fr = p.x + c2 * nr;                             // etd2: u^+
+      fi = p.y + c2 * ni;
+      if (id.valid) a.uout[i] = mk<double>(fr, fi);
     }
-    x[e] = mk<T>(T(fr), T(fi));
+    nf[e] = mk<double>(fr, fi);
   }
-  next_field_tail<T, N, TB>(x, j, id, g, lds, a.tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB>(nf, j, id, g, lds, a.tw, a.P, a.Q, a.part, a.dc, red);
+}
+
+// widening copy of a spectrum into the FP64 state
+template <typename T>
+__global__ void k_widen(const cpx<T>* __restrict__ in, cpx<double>* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = mk<double>(double(in[i].x), double(in[i].y));
 }
 
 }  // namespace vb

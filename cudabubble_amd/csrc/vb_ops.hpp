@@ -30,7 +30,8 @@ struct SizeOps {
   void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
   void (*zpass)(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
   void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s);
-  void (*zprep)(const cpx<T>* vh, cpx<T>* P, cpx<T>* Q, double* part, const Geo& g, const cpx<T>* tw, hipStream_t s);
+  void (*zprep)(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g, const cpx<T>* tw,
+                hipStream_t s);
   void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);
 };
 

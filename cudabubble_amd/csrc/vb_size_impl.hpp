@@ -60,9 +60,9 @@ struct Launch {
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s) {
     hipLaunchKernelGGL((k_zlap<T, N>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
-  static void zprep(const cpx<T>* vh, cpx<T>* P, cpx<T>* Q, double* part, const Geo& g, const cpx<T>* tw,
-                    hipStream_t s) {
-    hipLaunchKernelGGL((k_zprep<T, N>), dim3(zgrid(g)), dim3(YT), 0, s, vh, P, Q, part, g, tw);
+  static void zprep(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g,
+                    const cpx<T>* tw, hipStream_t s) {
+    hipLaunchKernelGGL((k_zprep<T, N>), dim3(zgrid(g)), dim3(YT), 0, s, vh, P, Q, part, dc, g, tw);
   }
   static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
     const dim3 gr(zgrid(a.g)), bl(YT);

@@ -28,6 +28,7 @@ namespace vb {
 struct FinArgs {
   const double* xpart;
   const double* zpart;
+  const double* dc;   // k = 0 coefficient of the field (its mean)
   Scal* sc;
   int nxb, nzb, mode, pad;
   Geo g;
@@ -52,8 +53,10 @@ __global__ void __launch_bounds__(256) k_finalize(FinArgs a) {
     const double L3 = a.g.L3;
     const double eps = a.ph.eps;
     // vol: intf(d, 1, 1/2); surface: intf(s, 0, 3/(2 sqrt2)) with
-    // s = eps/2 |grad d|^2 + (d^2-1)^2/(4 eps); elastic: intf(f^2, 0, 1/(2 eps))
-    const double V = (tot[0] / N + 1.0) * (0.5 * L3);
+    // s = eps/2 |grad d|^2 + (d^2-1)^2/(4 eps); elastic: intf(f^2, 0, 1/(2 eps)).
+    // mean(d) is the k = 0 coefficient of the FP64 spectrum (Parseval gives
+    // mean |grad d|^2 the same way); the spatial sum is kept as sum_v.
+    const double V = (a.dc[0] + 1.0) * (0.5 * L3);
     const double ms = 0.5 * eps * tot[3] + 0.25 / eps * (tot[1] / N);
     const double A = ms * (3.0 / (2.0 * sqrt(2.0)) * L3);
     const double Eb = (tot[2] / N) * (1.0 / (eps * 2) * L3);
@@ -89,11 +92,12 @@ __global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, 
   }
 }
 
-void launch_finalize(const double* xpart, int nxb, const double* zpart, int nzb, Scal* sc, const Geo& g,
-                     const Phys& ph, int mode, hipStream_t s) {
+void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
+                     const Geo& g, const Phys& ph, int mode, hipStream_t s) {
   FinArgs a;
   a.xpart = xpart;
   a.zpart = zpart;
+  a.dc = dc;
   a.sc = sc;
   a.nxb = nxb;
   a.nzb = nzb;
@@ -119,11 +123,20 @@ struct Solver final : SolverBase {
   bool own_stream = false;
   const SizeOps<T>* ox = nullptr;
   const SizeOps<T>* oz = nullptr;
-  DBuf<cpx<T>> uh[2], pre, acc, P, Q, Tt, W, S;
+  // FP64 operations (state transforms, FP64 queries in mixed mode)
+  const SizeOps<double>* oxd = nullptr;
+  const SizeOps<double>* ozd = nullptr;
+  DBuf<cpx<double>> uh[2], pre, acc, S;   // state and combine accumulators: FP64
+  DBuf<cpx<T>> P, Q, Tt, W;              // transform work buffers
   DBuf<T> v, f;
-  DBuf<double> xpart, zpart, phid;
+  DBuf<double> xpart, zpart, phid, dc;
   DBuf<Scal> scal;
   DBuf<cpx<T>> twx, twz;
+  DBuf<cpx<double>> twxd, twzd;
+  // FP64 evaluation of the integrals of the state (mixed precision only)
+  DBuf<cpx<double>> P64, Q64, T64;
+  DBuf<double> v64, f64, xpart64, zpart64, dc64;
+  DBuf<Scal> scal64;
   Scal* hs = nullptr;   // pinned host copy
   int nxb = 0, nzb = 0;
   long long npairs = 0;
@@ -160,6 +173,8 @@ struct Solver final : SolverBase {
     ph.M2 = p.M2;
     ox = get_ops<T>(p.nx);
     oz = get_ops<T>(p.nz);
+    oxd = get_ops<double>(p.nx);
+    ozd = get_ops<double>(p.nz);
     if (shared) {
       st = shared;
     } else {
@@ -167,8 +182,10 @@ struct Solver final : SolverBase {
       own_stream = true;
     }
     const size_t nc = (size_t)g.nc, nr = (size_t)g.nr;
-    for (auto* b : {&uh[0], &uh[1], &pre, &acc, &P, &Q, &Tt, &S}) b->alloc(nc);
+    for (auto* b : {&uh[0], &uh[1], &pre, &acc, &S}) b->alloc(nc);
+    for (auto* b : {&P, &Q, &Tt}) b->alloc(nc);
     if (p.scheme == VB_EULER || p.shape_adjust) W.alloc(nc);
+    dc.alloc(1);
     v.alloc(nr);
     f.alloc(nr);
     phid.alloc(nr);
@@ -183,6 +200,8 @@ struct Solver final : SolverBase {
     VB_HIP(hipMemsetAsync(scal.p, 0, sizeof(Scal), st));
     upload_twiddles(twx, p.nx);
     upload_twiddles(twz, p.nz);
+    upload_twiddles(twxd, p.nx);
+    upload_twiddles(twzd, p.nz);
     mmax = mmax_of(p.nx, p.ny, p.nz);
     time_step = p.time_step;
     alpha = p.alpha;
@@ -201,16 +220,17 @@ struct Solver final : SolverBase {
     if (own_stream && st) (void)hipStreamDestroy(st);
   }
 
-  void upload_twiddles(DBuf<cpx<T>>& b, int n) {
+  template <typename X>
+  void upload_twiddles(DBuf<cpx<X>>& b, int n) {
     std::vector<double> t;
     build_twiddles(t, n);
-    std::vector<cpx<T>> h((size_t)n);
+    std::vector<cpx<X>> h((size_t)n);
     for (int m = 0; m < n; ++m) {
-      h[(size_t)m].x = T(t[2 * (size_t)m]);
-      h[(size_t)m].y = T(t[2 * (size_t)m + 1]);
+      h[(size_t)m].x = X(t[2 * (size_t)m]);
+      h[(size_t)m].y = X(t[2 * (size_t)m + 1]);
     }
     b.alloc((size_t)n);
-    VB_HIP(hipMemcpy(b.p, h.data(), sizeof(cpx<T>) * (size_t)n, hipMemcpyHostToDevice));
+    VB_HIP(hipMemcpy(b.p, h.data(), sizeof(cpx<X>) * (size_t)n, hipMemcpyHostToDevice));
   }
 
   const Coef* lut(int sch, double h) {
@@ -295,8 +315,8 @@ struct Solver final : SolverBase {
   // ---- pipeline pieces ------------------------------------------------------
   void check_launch() { VB_HIP(hipGetLastError()); }
 
-  void prep(const cpx<T>* vh) {
-    L("zprep", [&] { oz->zprep(vh, P.p, Q.p, zpart.p, g, twz.p, st); });
+  void prep(const cpx<double>* vh) {
+    L("zprep", [&] { oz->zprep(vh, P.p, Q.p, zpart.p, dc.p, g, twz.p, st); });
     check_launch();
     prepped = true;
   }
@@ -325,7 +345,7 @@ struct Solver final : SolverBase {
     L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf, st); });
     spectral_lap(Tt.p);
     if (withf) spectral_lap(W.p);
-    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, nzb, scal.p, g, ph, mode, st); });
+    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
     check_launch();
     prepped = false;
     prefixed = true;
@@ -333,7 +353,7 @@ struct Solver final : SolverBase {
   }
 
   void refinalize(int mode) {
-    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, nzb, scal.p, g, ph, mode, st); });
+    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
     check_launch();
   }
 
@@ -361,6 +381,7 @@ struct Solver final : SolverBase {
     z.P = P.p;
     z.Q = Q.p;
     z.part = zpart.p;
+    z.dc = dc.p;
     z.lut = lt;
     z.tw = twz.p;
     z.g = g;
@@ -400,7 +421,7 @@ struct Solver final : SolverBase {
     e.W = W.p;
     e.v = v.p;
     e.f = f.p;
-    e.out = uh[out].p;
+    e.out = W.p;   // in place over the rows it reads
     e.sc = scal.p;
     e.tw = twx.p;
     e.npairs = npairs;
@@ -410,9 +431,13 @@ struct Solver final : SolverBase {
     e.inv_n = 1.0 / double(g.nr);
     e.shape = shape ? 1 : 0;
     L("xe", [&] { ox->xe(e, st); });
-    L("y_fwd", [&] { ox->ypass(uh[out].p, nullptr, 1, -1, g, twx.p, st); });
-    L("z_fwd", [&] { oz->zpass(uh[out].p, -1, g, twz.p, st); });
+    L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, st); });
+    L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
+    L("widen", [&] {
+      hipLaunchKernelGGL(k_widen<T>, dim3(unsigned((g.nc + 255) / 256)), dim3(256), 0, st, W.p, uh[out].p, g.nc);
+    });
     check_launch();
+    prefixed_f = false;   // W was consumed
     return out;
   }
 
@@ -606,9 +631,54 @@ struct Solver final : SolverBase {
     if (rep) *rep = r;
   }
 
+  // Integrals of the state evaluated entirely in FP64 (mixed precision):
+  // FP64 z, y and x transforms of the FP64 state, FP64 f and reductions.
+  void integrals64() {
+    const size_t nc = (size_t)g.nc, nr = (size_t)g.nr;
+    if (!P64.p) {
+      P64.alloc(nc);
+      Q64.alloc(nc);
+      T64.alloc(nc);
+      v64.alloc(nr);
+      f64.alloc(nr);
+      dc64.alloc(1);
+      scal64.alloc(1);
+    }
+    const int nxb64 = int((npairs + oxd->xlanes - 1) / oxd->xlanes);
+    const int nzb64 = int(((long long)g.ny * g.nh + ozd->ytile - 1) / ozd->ytile);
+    if (xpart64.n < (size_t)nxb64 * 3) xpart64.alloc((size_t)nxb64 * 3);
+    if (zpart64.n < (size_t)nzb64) zpart64.alloc((size_t)nzb64);
+    L("zprep64", [&] { ozd->zprep(uh[cur].p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
+    L("y_inv2_64", [&] { oxd->ypass(P64.p, Q64.p, 2, +1, g, twxd.p, st); });
+    XaArgs<double> a;
+    a.P = P64.p;
+    a.Q = Q64.p;
+    a.v = v64.p;
+    a.f = f64.p;
+    a.Tt = T64.p;
+    a.W = nullptr;
+    a.part = xpart64.p;
+    a.tw = twxd.p;
+    a.npairs = npairs;
+    a.eps = p.eps;
+    a.kappa = p.kappa;
+    a.inv_n = 1.0 / double(g.nr);
+    L("xa64", [&] { oxd->xa(a, false, st); });
+    VB_HIP(hipMemcpyAsync(scal64.p, scal.p, sizeof(Scal), hipMemcpyDeviceToDevice, st));
+    L("finalize", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
+    check_launch();
+    VB_HIP(hipMemcpyAsync(hs, scal64.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+  }
+
   void query(double* E, double* V, double* A, double* Eb) override {
-    ensure_prefixed();
-    read_scal();
+    if (sizeof(T) == sizeof(double)) {
+      ensure_prefixed();
+      read_scal();
+    } else {
+      if (started && p.penalty_drift) sync_targets_from_device();
+      integrals64();
+    }
     if (E) *E = hs->E;
     if (V) *V = hs->V;
     if (A) *A = hs->A;
@@ -636,10 +706,10 @@ struct Solver final : SolverBase {
   }
 
   void get_phi(double* host) override {
-    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<T>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));
-    L("z_inv", [&] { oz->zpass(S.p, +1, g, twz.p, st); });
-    L("y_inv", [&] { ox->ypass(S.p, nullptr, 1, +1, g, twx.p, st); });
-    L("x_c2r", [&] { ox->xc2r_d(S.p, phid.p, npairs, twx.p, st); });
+    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<double>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));
+    L("z_inv", [&] { ozd->zpass(S.p, +1, g, twzd.p, st); });
+    L("y_inv", [&] { oxd->ypass(S.p, nullptr, 1, +1, g, twxd.p, st); });
+    L("x_c2r", [&] { oxd->xc2r_d(S.p, phid.p, npairs, twxd.p, st); });
     check_launch();
     VB_HIP(hipMemcpyAsync(host, phid.p, sizeof(double) * (size_t)g.nr, hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
@@ -647,9 +717,9 @@ struct Solver final : SolverBase {
 
   void set_phi(const double* host) override {
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
-    L("x_r2c", [&] { ox->xr2c_d(phid.p, uh[cur].p, npairs, twx.p, 1.0 / double(g.nr), st); });
-    L("y_fwd", [&] { ox->ypass(uh[cur].p, nullptr, 1, -1, g, twx.p, st); });
-    L("z_fwd", [&] { oz->zpass(uh[cur].p, -1, g, twz.p, st); });
+    L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), st); });
+    L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, st); });
+    L("z_fwd", [&] { ozd->zpass(uh[cur].p, -1, g, twzd.p, st); });
     check_launch();
     VB_HIP(hipStreamSynchronize(st));
     prepped = prefixed = prefixed_f = false;
@@ -669,10 +739,13 @@ std::unique_ptr<SolverBase> make_solver(const vb_params& p, hipStream_t shared) 
 size_t solver_footprint(const vb_params& p) {
   const size_t nh = (size_t)p.nx / 2 + 1;
   const size_t nc = nh * (size_t)p.nx * (size_t)p.nz, nr = (size_t)p.nx * p.nx * p.nz;
-  const size_t cs = p.precision == VB_FP32_MIXED ? 8 : 16, rs = cs / 2;
-  size_t ncb = 8 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
+  const bool mixed = p.precision == VB_FP32_MIXED;
+  const size_t cs = mixed ? 8 : 16, rs = cs / 2;
+  const size_t work = 3 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
+  size_t bytes = 5 * nc * 16 + work * nc * cs + 2 * nr * rs + nr * 8;
+  if (mixed) bytes += 3 * nc * 16 + 2 * nr * 8;   // FP64 query buffers
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
-  return ncb * nc * cs + 2 * nr * rs + nr * 8 + 2 * mm * sizeof(Coef) + (size_t)4 << 20;
+  return bytes + 2 * mm * sizeof(Coef) + ((size_t)4 << 20);
 }
 
 }  // namespace vb

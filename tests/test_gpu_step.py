@@ -27,12 +27,16 @@ def oracle_for(oracle_mod, p):
                                            (32, 32, 2), (64, 64, 2)])
 def test_single_evaluation(vb, gpu, oracle_mod, nx, nz, scheme):
     flush_print("evaluate", nx, nz, scheme)
-    p, phi = make(vb, nx, nz, M1=1e4, M2=1e4, alpha=20.0, beta=40.0, scheme=scheme)
+    p, phi = make(vb, nx, nz, M1=1e4, M2=1e4, scheme=scheme)
+    o0 = oracle_mod.Oracle(nx, nz, eps=p.eps)
+    # targets 1% off the shape's own V and A, so the multipliers are active but bounded
+    p.alpha, p.beta = 1.01 * o0.vol(phi), 0.99 * o0.surface(phi)
     o = oracle_for(oracle_mod, p)
     s = vb.Solver(p, phi)
-    for h in (1e-7, 2e-6):
+    for h in (1e-7, 5e-7):
         s.evaluate(h)
         phi = o.etd4rk(phi, h) if scheme == 4 else o.etd2rk(phi, h)
+        assert np.abs(phi).max() < 2.0
         assert rel_l2(s.get_phi(), phi) < 1e-12, h
 
 
