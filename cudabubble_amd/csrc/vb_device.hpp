@@ -15,6 +15,14 @@ struct Geo {
   double kfac;          // 2 pi / L
   double kfac2;         // (2 pi / L)^2
   double L3;            // L^3
+  // Row pitch of half spectra in elements.  pitch = nh is the reference
+  // layout; the solver pads rows to nx/2 + 16 so every row starts on a
+  // 128-byte boundary, and then (aligned = 1) the y and z kernels tile
+  // kx in [0, nx/2) with aligned column groups and treat the Nyquist column
+  // kx = nx/2 as a separate group.
+  int pitch, aligned;
+  long long pp;         // plane pitch (z stride) of spectra in elements
+  long long ncp;        // allocated spectrum elements, nz * pp
 };
 
 // Physical constants of one solve (reference names).

@@ -122,43 +122,53 @@ template <int S, typename T> struct Dft<8, S, T> {
 // ---------------------------------------------------------------------------
 // Stockham passes.  Lds is a functor: lds(pos) -> cpx<T>& for this lane.
 // ---------------------------------------------------------------------------
+// Butterflies of pass P for one transform held in x.  Non-last passes write
+// their outputs to LDS; the last pass leaves them in y (same positions).
+template <typename T, int N, int S, int P, class Lds>
+__device__ __forceinline__ void pass_body(const cpx<T>* x, cpx<T>* y, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
+  constexpr int E = Plan<N>::E;
+  constexpr int TF = N / E;
+  constexpr int R = Plan<N>::R[P];
+  constexpr int NS = ns_before<N, P>();
+  constexpr int ER = E / R;
+  constexpr bool last = (P == Plan<N>::NP - 1);
+  constexpr int TST = N / (NS * R);
+#pragma unroll
+  for (int t = 0; t < ER; ++t) {
+    const int b = j + TF * t;
+    const int k = b % NS;
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = x[t + r * ER];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> w = tw[r * k * TST];
+        if constexpr (S > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    Dft<R, S, T>::run(v);
+    if constexpr (last) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) y[t + q * ER] = v[q];
+    } else {
+      const int base = (b / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds(base + q * NS) = v[q];
+    }
+  }
+}
+
 template <typename T, int N, int S, int P>
 struct Pass {
   template <class Lds>
   static __device__ __forceinline__ void run(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
     constexpr int E = Plan<N>::E;
     constexpr int TF = N / E;
-    constexpr int R = Plan<N>::R[P];
-    constexpr int NS = ns_before<N, P>();
-    constexpr int ER = E / R;
     constexpr bool last = (P == Plan<N>::NP - 1);
-    constexpr int TST = N / (NS * R);
     cpx<T> y[last ? E : 1];
-#pragma unroll
-    for (int t = 0; t < ER; ++t) {
-      const int b = j + TF * t;
-      const int k = b % NS;
-      cpx<T> v[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = x[t + r * ER];
-      if constexpr (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[r * k * TST];
-          if constexpr (S > 0) w.y = -w.y;
-          v[r] = cmul(v[r], w);
-        }
-      }
-      Dft<R, S, T>::run(v);
-      if constexpr (last) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) y[t + q * ER] = v[q];
-      } else {
-        const int base = (b / NS) * NS * R + k;
-#pragma unroll
-        for (int q = 0; q < R; ++q) lds(base + q * NS) = v[q];
-      }
-    }
+    pass_body<T, N, S, P>(x, y, j, tw, lds);
     if constexpr (last) {
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = y[e];
@@ -172,12 +182,51 @@ struct Pass {
   }
 };
 
+// Two independent transforms through the same passes: one set of barriers
+// for both, and twice the independent work between barriers.
+template <typename T, int N, int S, int P>
+struct Pass2 {
+  template <class Lds>
+  static __device__ __forceinline__ void run(cpx<T>* x0, cpx<T>* x1, int j, const cpx<T>* __restrict__ tw,
+                                             Lds& l0, Lds& l1) {
+    constexpr int E = Plan<N>::E;
+    constexpr int TF = N / E;
+    constexpr bool last = (P == Plan<N>::NP - 1);
+    cpx<T> y0[last ? E : 1], y1[last ? E : 1];
+    pass_body<T, N, S, P>(x0, y0, j, tw, l0);
+    pass_body<T, N, S, P>(x1, y1, j, tw, l1);
+    if constexpr (last) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        x0[e] = y0[e];
+        x1[e] = y1[e];
+      }
+    } else {
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        x0[e] = l0(j + TF * e);
+        x1[e] = l1(j + TF * e);
+      }
+      __syncthreads();
+      Pass2<T, N, S, P + 1>::run(x0, x1, j, tw, l0, l1);
+    }
+  }
+};
+
 // Full 1-D transform (unnormalised): S = -1 forward, +1 inverse.
 // Contains (NP - 1) * 2 workgroup barriers; every thread of the workgroup
 // must call it the same number of times.
 template <typename T, int N, int S, class Lds>
 __device__ __forceinline__ void fft(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
   Pass<T, N, S, 0>::run(x, j, tw, lds);
+}
+
+// Two transforms of the same length and direction, each with its own LDS view.
+template <typename T, int N, int S, class Lds>
+__device__ __forceinline__ void fft2(cpx<T>* x0, cpx<T>* x1, int j, const cpx<T>* __restrict__ tw, Lds& l0,
+                                     Lds& l1) {
+  Pass2<T, N, S, 0>::run(x0, x1, j, tw, l0, l1);
 }
 
 // LDS view for a lane whose transform is stored contiguously (x kernels):

@@ -43,6 +43,13 @@ __device__ __forceinline__ cpx<T> pack2(const cpx<T>* __restrict__ A, const cpx<
   return mk<T>(a.x - b.y, a.y + b.x);
 }
 
+// Layout of half-spectrum rows: row r = z * ny + y starts at z * pp + y * pitch.
+struct Rows {
+  int ny, pitch;
+  long long pp;
+  __host__ __device__ long long off(long long r) const { return (r / ny) * pp + (r % ny) * pitch; }
+};
+
 // After a forward transform of Z = a + i b (a, b real rows), write the half
 // spectra of a and b, times s.  Uses the lane's LDS; ends with a barrier.
 template <typename T, int N, class Lds>
@@ -72,7 +79,8 @@ __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, c
 // ---------------------------------------------------------------------------
 template <typename TI, typename T, int N>
 __global__ void __launch_bounds__(XWG) k_xr2c(const TI* __restrict__ in, cpx<T>* __restrict__ out,
-                                               long long npairs, const cpx<T>* __restrict__ tw, double scl) {
+                                               long long npairs, const cpx<T>* __restrict__ tw, double scl,
+                                               Rows rl) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
@@ -88,12 +96,13 @@ __global__ void __launch_bounds__(XWG) k_xr2c(const TI* __restrict__ in, cpx<T>*
     x[e] = mk<T>(T(in[r0 * N + p]), T(in[(r0 + 1) * N + p]));
   }
   fft<T, N, -1>(x, j, tw, lds);
-  unpack_store<T, N>(x, j, lds, out + r0 * NH, out + (r0 + 1) * NH, T(scl), valid);
+  const long long o0 = rl.off(r0);
+  unpack_store<T, N>(x, j, lds, out + o0, out + o0 + rl.pitch, T(scl), valid);
 }
 
 template <typename T, typename TO, int N>
 __global__ void __launch_bounds__(XWG) k_xc2r(const cpx<T>* __restrict__ in, TO* __restrict__ out,
-                                               long long npairs, const cpx<T>* __restrict__ tw) {
+                                               long long npairs, const cpx<T>* __restrict__ tw, Rows rl) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
@@ -103,8 +112,9 @@ __global__ void __launch_bounds__(XWG) k_xc2r(const cpx<T>* __restrict__ in, TO*
   if (!valid) pair = npairs - 1;
   const long long r0 = 2 * pair;
   cpx<T> x[E];
+  const long long o0 = rl.off(r0);
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(in + r0 * NH, in + (r0 + 1) * NH, j + TF * e);
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(in + o0, in + o0 + rl.pitch, j + TF * e);
   fft<T, N, +1>(x, j, tw, lds);
   if (valid) {
 #pragma unroll
@@ -131,15 +141,17 @@ struct XaArgs {
   const cpx<T>* tw;
   long long npairs;
   double eps, kappa, inv_n;
+  Rows rl;
 };
 
 template <typename T, int N, bool WITHF>
 __global__ void __launch_bounds__(XWG) k_xa(XaArgs<T> a) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
-  __shared__ cpx<T> sm[G * LL];
+  __shared__ cpx<T> sm[2 * G * LL];
   __shared__ double red[3 * (XWG / 64)];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
+  RowLds<T> lds2{sm + (G + lane) * LL};
   long long pair = (long long)blockIdx.x * G + lane;
   const bool valid = pair < a.npairs;
   if (!valid) pair = a.npairs - 1;
@@ -148,27 +160,33 @@ __global__ void __launch_bounds__(XWG) k_xa(XaArgs<T> a) {
   double acc[3] = {0.0, 0.0, 0.0};
   T tm[2][E];
   T ff[WITHF ? 2 : 1][E];
+  {
+    cpx<T> x[2][E];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const long long r = r0 + h;
-    cpx<T> x[E];
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + r * NH, a.Q + r * NH, j + TF * e);
-    fft<T, N, +1>(x, j, a.tw, lds);
+      for (int e = 0; e < E; ++e)
+        x[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.Q + a.rl.off(r0 + h), j + TF * e);
+    }
+    fft2<T, N, +1>(x[0], x[1], j, a.tw, lds, lds2);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int p = j + TF * e;
-      const double v = double(x[e].x), lv = double(x[e].y);
-      const double w = v * v - 1.0;
-      const double f = -lv * eps + w * v / eps;          // f_kernel, Calc.cu:160
-      const double t1 = v * (v * v - 1.0 - kappa);       // nld_kernel1, Calc.cu:270
-      if (valid) {
-        acc[0] += v; acc[1] += w * w; acc[2] += f * f;
-        a.v[r * N + p] = T(v);
-        a.f[r * N + p] = T(f);
+    for (int h = 0; h < 2; ++h) {
+      const long long r = r0 + h;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int p = j + TF * e;
+        const double v = double(x[h][e].x), lv = double(x[h][e].y);
+        const double w = v * v - 1.0;
+        const double f = -lv * eps + w * v / eps;          // f_kernel, Calc.cu:160
+        const double t1 = v * (v * v - 1.0 - kappa);       // nld_kernel1, Calc.cu:270
+        if (valid) {
+          acc[0] += v; acc[1] += w * w; acc[2] += f * f;
+          a.v[r * N + p] = T(v);
+          a.f[r * N + p] = T(f);
+        }
+        tm[h][e] = T(t1);
+        if constexpr (WITHF) ff[h][e] = T(f);
       }
-      tm[h][e] = T(t1);
-      if constexpr (WITHF) ff[h][e] = T(f);
     }
   }
   {
@@ -176,14 +194,14 @@ __global__ void __launch_bounds__(XWG) k_xa(XaArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
     fft<T, N, -1>(z, j, a.tw, lds);
-    unpack_store<T, N>(z, j, lds, a.Tt + r0 * NH, a.Tt + (r0 + 1) * NH, T(a.inv_n), valid);
+    unpack_store<T, N>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   if constexpr (WITHF) {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(ff[0][e], ff[1][e]);
     fft<T, N, -1>(z, j, a.tw, lds);
-    unpack_store<T, N>(z, j, lds, a.W + r0 * NH, a.W + (r0 + 1) * NH, T(a.inv_n), valid);
+    unpack_store<T, N>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
@@ -200,6 +218,7 @@ struct XbArgs {
   const cpx<T>* tw;
   long long npairs;
   double eps, kappa, D, eta, inv_n;
+  Rows rl;
 };
 
 template <typename T, int N>
@@ -213,8 +232,17 @@ __global__ void __launch_bounds__(XWG) k_xb(XbArgs<T> a) {
   if (!valid) pair = a.npairs - 1;
   const long long r0 = 2 * pair;
   cpx<T> x[E];
+  T vv[2][E], ffv[2][E];   // prefetched before the transform so the loads overlap it
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.Tt + r0 * NH, a.Tt + (r0 + 1) * NH, j + TF * e);
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      vv[h][e] = a.v[(r0 + h) * N + j + TF * e];
+      ffv[h][e] = a.f[(r0 + h) * N + j + TF * e];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, j + TF * e);
   fft<T, N, +1>(x, j, a.tw, lds);
   const double l1 = a.sc->l1, l2 = a.sc->l2;
   const double ie = 1.0 / a.eps, ie2 = ie * ie, ie3 = ie * ie2;
@@ -226,9 +254,8 @@ __global__ void __launch_bounds__(XWG) k_xb(XbArgs<T> a) {
     double nn[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const long long r = r0 + h;
       const double dt1 = double(h ? x[e].y : x[e].x);
-      const double v = double(a.v[r * N + p]), f = double(a.f[r * N + p]);
+      const double v = double(vv[h][e]), f = double(ffv[h][e]);
       const double t1 = v * (v * v - 1.0 - kappa);
       const double nl = -dt1 * ie + D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f;   // Calc.cu:282
       nn[h] = -l1 * 0.5 - l2 * f * 3.0 / c32 - nl * eta;                           // Calc.cu:435
@@ -236,7 +263,7 @@ __global__ void __launch_bounds__(XWG) k_xb(XbArgs<T> a) {
     x[e] = mk<T>(T(nn[0]), T(nn[1]));
   }
   fft<T, N, -1>(x, j, a.tw, lds);
-  unpack_store<T, N>(x, j, lds, a.Tt + r0 * NH, a.Tt + (r0 + 1) * NH, T(a.inv_n), valid);
+  unpack_store<T, N>(x, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -253,6 +280,7 @@ struct XeArgs {
   long long npairs;
   double eps, eta, t, inv_n;
   int shape;         // 1: shape_adjust (rhs = g0), 0: run_static (rhs = nld1)
+  Rows rl;
 };
 
 template <typename T, int N>
@@ -267,7 +295,7 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
   const long long r0 = 2 * pair;
   cpx<T> x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + r0 * NH, a.W + (r0 + 1) * NH, j + TF * e);
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, j + TF * e);
   fft<T, N, +1>(x, j, a.tw, lds);
   const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.t;
 #pragma unroll
@@ -286,34 +314,69 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
     x[e] = mk<T>(T(nv[0]), T(nv[1]));
   }
   fft<T, N, -1>(x, j, a.tw, lds);
-  unpack_store<T, N>(x, j, lds, a.out + r0 * NH, a.out + (r0 + 1) * NH, T(a.inv_n), valid);
+  unpack_store<T, N>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------
 // y kernels: complex transforms along y, in place, batch over (z, kx).
 // blockIdx.y selects one of two arrays.
 // ---------------------------------------------------------------------------
+// Column of one y transform: (z, kx) -> base address; rows are pitch apart.
+struct YIdx {
+  long long base;
+  bool valid;
+};
+
+template <int TB>
+__device__ __forceinline__ YIdx yidx(const Geo& g) {
+  const int c = threadIdx.x % TB;
+  YIdx r;
+  int z, kx;
+  if (g.aligned) {
+    // the Nyquist groups (uncoalesced, slower) come first so they overlap
+    // with the main groups instead of trailing them
+    const int tpr = (g.nx / 2) / TB;
+    const long long nnyq = (g.nz + TB - 1) / TB;
+    const long long t = blockIdx.x;
+    if (t >= nnyq) {
+      z = int((t - nnyq) / tpr);
+      kx = int((t - nnyq) % tpr) * TB + c;
+      r.valid = true;
+    } else {   // Nyquist column of TB planes
+      z = int(t * TB + c);
+      kx = g.nx / 2;
+      r.valid = z < g.nz;
+      if (!r.valid) z = g.nz - 1;
+    }
+  } else {
+    const long long batch = (long long)g.nz * g.nh;
+    long long b = (long long)blockIdx.x * TB + c;
+    r.valid = b < batch;
+    if (!r.valid) b = batch - 1;
+    z = int(b / g.nh);
+    kx = int(b % g.nh);
+  }
+  r.base = (long long)z * g.pp + kx;
+  return r;
+}
+
 template <typename T, int N, int S, int TB = tbw<T, N>()>
 __global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(cpx<T>* b0, cpx<T>* b1, Geo g,
                                                           const cpx<T>* __restrict__ tw) {
   constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   cpx<T>* buf = blockIdx.y ? b1 : b0;
-  const int c = threadIdx.x % TB, j = threadIdx.x / TB;
-  ColLds<T, TB> lds{sm + c};
-  const long long batch = (long long)g.nz * g.nh;
-  long long b = (long long)blockIdx.x * TB + c;
-  const bool valid = b < batch;
-  if (!valid) b = batch - 1;
-  const long long z = b / g.nh, kx = b % g.nh;
-  cpx<T>* col = buf + z * (long long)N * g.nh + kx;
+  const int j = threadIdx.x / TB;
+  ColLds<T, TB> lds{sm + threadIdx.x % TB};
+  const YIdx id = yidx<TB>(g);
+  cpx<T>* col = buf + id.base;
   cpx<T> x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = col[(long long)(j + TF * e) * g.nh];
+  for (int e = 0; e < E; ++e) x[e] = col[(long long)(j + TF * e) * g.pitch];
   fft<T, N, S>(x, j, tw, lds);
-  if (valid) {
+  if (id.valid) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.nh] = x[e];
+    for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
   }
 }
 
@@ -328,15 +391,43 @@ struct ZIdx {
 
 template <int TB>
 __device__ __forceinline__ ZIdx zidx(const Geo& g) {
+  const int c = threadIdx.x % TB;
   ZIdx r;
-  const long long plane = (long long)g.ny * g.nh;
-  long long q = (long long)blockIdx.x * TB + (threadIdx.x % TB);
-  r.valid = q < plane;
-  if (!r.valid) q = plane - 1;
-  r.q = q;
-  r.ky = int(q / g.nh);
-  r.kx = int(q % g.nh);
+  if (g.aligned) {
+    const int tpr = (g.nx / 2) / TB;
+    const long long nnyq = (g.ny + TB - 1) / TB;
+    const long long t = blockIdx.x;
+    if (t >= nnyq) {
+      r.ky = int((t - nnyq) / tpr);
+      r.kx = int((t - nnyq) % tpr) * TB + c;
+      r.valid = true;
+    } else {   // Nyquist column of TB rows (scheduled first, see yidx)
+      int ky = int(t * TB + c);
+      r.valid = ky < g.ny;
+      if (!r.valid) ky = g.ny - 1;
+      r.ky = ky;
+      r.kx = g.nx / 2;
+    }
+  } else {
+    const long long ncol = (long long)g.ny * g.nh;
+    long long q = (long long)blockIdx.x * TB + c;
+    r.valid = q < ncol;
+    if (!r.valid) q = ncol - 1;
+    r.ky = int(q / g.nh);
+    r.kx = int(q % g.nh);
+  }
+  r.q = (long long)r.ky * g.pitch + r.kx;
   return r;
+}
+
+// number of y / z workgroups for a tile width
+__host__ __device__ inline long long ygroups(const Geo& g, int TB) {
+  if (g.aligned) return (long long)g.nz * ((g.nx / 2) / TB) + (g.nz + TB - 1) / TB;
+  return ((long long)g.nz * g.nh + TB - 1) / TB;
+}
+__host__ __device__ inline long long zgroups(const Geo& g, int TB) {
+  if (g.aligned) return (long long)g.ny * ((g.nx / 2) / TB) + (g.ny + TB - 1) / TB;
+  return ((long long)g.ny * g.nh + TB - 1) / TB;
 }
 
 // |k|^2 of the gradient with each component's own Nyquist plane zeroed
@@ -361,7 +452,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zpass(cpx<T>* buf, Geo g, 
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
-  const long long plane = (long long)g.ny * g.nh;
+  const long long plane = g.pp;
   cpx<T>* col = buf + id.q;
   cpx<T> x[E];
 #pragma unroll
@@ -381,7 +472,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(cpx<T>* buf, Geo g, c
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
-  const long long plane = (long long)g.ny * g.nh;
+  const long long plane = g.pp;
   cpx<T>* col = buf + id.q;
   cpx<T> x[E];
 #pragma unroll
@@ -406,10 +497,10 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(cpx<T>* buf, Geo g, c
 // in FP64; then P = z-inverse(nf), Q = z-inverse(-|k|^2 nf) in T.
 template <typename T, int N, int TB, class Lds>
 __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, const ZIdx& id, const Geo& g, Lds& lds,
-                                                const cpx<T>* __restrict__ tw, cpx<T>* P, cpx<T>* Q,
+                                                Lds& lds2, const cpx<T>* __restrict__ tw, cpx<T>* P, cpx<T>* Q,
                                                 double* part, double* dc, double* red) {
   constexpr int E = Plan<N>::E, TF = N / E;
-  const long long plane = (long long)g.ny * g.nh;
+  const long long plane = g.pp;
   double acc[1] = {0.0};
   cpx<T> x[E], y[E];
 #pragma unroll
@@ -421,65 +512,70 @@ __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, co
     x[e] = mk<T>(T(re), T(im));
     y[e] = mk<T>(T(re * m), T(im * m));
   }
-  if (id.q == 0 && j == 0) dc[0] = nf[0].x;
-  fft<T, N, +1>(x, j, tw, lds);
+  if (id.valid && id.q == 0 && j == 0) dc[0] = nf[0].x;
+  fft2<T, N, +1>(x, y, j, tw, lds, lds2);
   if (id.valid) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) P[id.q + (long long)(j + TF * e) * plane] = x[e];
-  }
-  fft<T, N, +1>(y, j, tw, lds);
-  if (id.valid) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) Q[id.q + (long long)(j + TF * e) * plane] = y[e];
+    for (int e = 0; e < E; ++e) {
+      P[id.q + (long long)(j + TF * e) * plane] = x[e];
+      Q[id.q + (long long)(j + TF * e) * plane] = y[e];
+    }
   }
   block_sum_store<1, TB * TF>(acc, red, part + blockIdx.x);
 }
 
-template <typename T, int N>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zprep(const cpx<double>* __restrict__ vh, cpx<T>* P,
-                                                                   cpx<T>* Q, double* part, double* dc, Geo g,
-                                                                   const cpx<T>* __restrict__ tw) {
-  constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
-  __shared__ cpx<T> sm[N * TB];
+template <typename T, int N, int TB = tbw<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(const cpx<double>* __restrict__ vh, cpx<T>* P, cpx<T>* Q,
+                                                          double* part, double* dc, Geo g,
+                                                          const cpx<T>* __restrict__ tw) {
+  constexpr int E = Plan<N>::E, TF = N / E;
+  __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
+  ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
-  const long long plane = (long long)g.ny * g.nh;
+  const long long plane = g.pp;
   cpx<double> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = vh[id.q + (long long)(j + TF * e) * plane];
-  next_field_tail<T, N, TB>(x, j, id, g, lds, tw, P, Q, part, dc, red);
+  next_field_tail<T, N, TB>(x, j, id, g, lds, lds2, tw, P, Q, part, dc, red);
 }
 
 template <typename T>
 struct ZsArgs {
   const cpx<T>* U;        // x,y-forward of N (normalised)
-  const cpx<double>* uh;  // state spectrum
-  cpx<double>* pre;       // stage precombination
-  cpx<double>* acc;       // final-combine accumulator
+  const cpx<double>* uh;  // state spectrum u^
+  cpx<T>* pre;            // ETDRK4: e1 a^ - q N^u (stage 1 -> 3)
+  cpx<T>* Na;             // ETDRK4: N^a (stage 2 -> 4); ETDRK2: N^u (stage 1 -> 2)
+  cpx<T>* Nb;             // ETDRK4: N^b (stage 3 -> 4)
   cpx<double>* uout;      // new state spectrum (last stage)
   cpx<T>* P;
   cpx<T>* Q;
   double* part;
   double* dc;
-  const Coef* lut;
+  const double2* c1;      // per m: ETDRK4 {e1, (e1-1)/H}; ETDRK2 {e^z, (e^z-1)/H}
+  const double4* c2;      // per m: ETDRK4 {e2, A s, 2 B s, C s}; ETDRK2 {c2, 0, 0, 0}
   const cpx<T>* tw;
   Geo g;
 };
 
 // SCH = 4 (ETDRK4, belastic.cpp:519-552 / Calc.cu:488-577) or 2 (ETDRK2,
 // belastic.cpp:491-517 / Calc.cu:446-486); STG = stage whose N^ this is.
-template <typename T, int N, int SCH, int STG>
-__global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zstage(ZsArgs<T> a) {
-  constexpr int E = Plan<N>::E, TF = N / E, TB = tbw<T, N>();
-  __shared__ cpx<T> sm[N * TB];
+// The N^ of a stage leaves the FFT in precision T and is stored as such, so
+// keeping it for a later stage loses nothing; the state and every combine
+// that produces it are FP64.
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a) {
+  constexpr int E = Plan<N>::E, TF = N / E;
+  __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
+  ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g);
-  const long long plane = (long long)g.ny * g.nh;
+  const long long plane = g.pp;
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
@@ -489,52 +585,55 @@ __global__ void __launch_bounds__((tbw<T, N>() * TFof<N>())) k_zstage(ZsArgs<T> 
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
     const long long i = id.q + (long long)kz * plane;
-    const Coef& cf = a.lut[mindex(g, id.kx, id.ky, kz)];
+    const int m = mindex(g, id.kx, id.ky, kz);
     const double nr = double(x[e].x), ni = double(x[e].y);
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
-      const double e1 = cf.c[0], q = cf.c[1];
+      const double2 c = a.c1[m];                      // e1, q
       const cpx<double> u = a.uh[i];
-      fr = e1 * u.x + q * nr;                         // etd41: a^
-      fi = e1 * u.y + q * ni;
-      if (id.valid) a.pre[i] = mk<double>(e1 * fr - q * nr, e1 * fi - q * ni);
+      fr = c.x * u.x + c.y * nr;                      // etd41: a^
+      fi = c.x * u.y + c.y * ni;
+      if (id.valid) a.pre[i] = mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni));
     } else if constexpr (SCH == 4 && STG == 2) {
-      const double e1 = cf.c[0], q = cf.c[1], e2 = cf.c[2], fa = cf.c[3], fb = cf.c[4];
+      const double2 c = a.c1[m];
       const cpx<double> u = a.uh[i];
-      fr = e1 * u.x + q * nr;                         // etd42: b^
-      fi = e1 * u.y + q * ni;
-      if (id.valid) a.acc[i] = mk<double>(e2 * u.x + fa * nr + fb * nr, e2 * u.y + fa * ni + fb * ni);
+      fr = c.x * u.x + c.y * nr;                      // etd42: b^
+      fi = c.x * u.y + c.y * ni;
+      if (id.valid) a.Na[i] = x[e];
     } else if constexpr (SCH == 4 && STG == 3) {
-      const double q = cf.c[1], fb = cf.c[4];
-      const cpx<double> p = a.pre[i];
-      fr = p.x + 2.0 * q * nr;                        // etd43: c^
-      fi = p.y + 2.0 * q * ni;
-      if (id.valid) {
-        const cpx<double> ac = a.acc[i];
-        a.acc[i] = mk<double>(ac.x + fb * nr, ac.y + fb * ni);
-      }
+      const double q = a.c1[m].y;
+      const cpx<T> p = a.pre[i];
+      fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
+      fi = double(p.y) + 2.0 * q * ni;
+      if (id.valid) a.Nb[i] = x[e];
     } else if constexpr (SCH == 4 && STG == 4) {
-      const double fc = cf.c[5];
-      const cpx<double> ac = a.acc[i];
-      fr = ac.x + fc * nr;                            // etd44: u^+
-      fi = ac.y + fc * ni;
+      const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
+      const cpx<double> u = a.uh[i];
+      const cpx<T> na = a.Na[i], nb = a.Nb[i];
+      // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
+      fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
+      fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
     } else if constexpr (SCH == 2 && STG == 1) {
-      const double ez = cf.c[0], q1 = cf.c[1], c2 = cf.c[2];
+      const double2 c = a.c1[m];                      // e^z, q1
       const cpx<double> u = a.uh[i];
-      fr = ez * u.x + q1 * nr;                        // etd1: a^
-      fi = ez * u.y + q1 * ni;
-      if (id.valid) a.pre[i] = mk<double>(fr - c2 * nr, fi - c2 * ni);
+      fr = c.x * u.x + c.y * nr;                      // etd1: a^
+      fi = c.x * u.y + c.y * ni;
+      if (id.valid) a.Na[i] = x[e];
     } else {                                          // SCH == 2 && STG == 2
-      const double c2 = cf.c[2];
-      const cpx<double> p = a.pre[i];
-      fr = p.x + c2 * nr;                             // etd2: u^+
-      fi = p.y + c2 * ni;
+      const double2 c = a.c1[m];
+      const double c2 = a.c2[m].x;
+      const cpx<double> u = a.uh[i];
+      const cpx<T> nu = a.Na[i];
+      // etd2: u^+ = F(a) + c2 (N^a - N^u), a^ = e^z u^ + q1 N^u
+      const double ar = c.x * u.x + c.y * double(nu.x), ai = c.x * u.y + c.y * double(nu.y);
+      fr = ar + c2 * (nr - double(nu.x));
+      fi = ai + c2 * (ni - double(nu.y));
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
     }
     nf[e] = mk<double>(fr, fi);
   }
-  next_field_tail<T, N, TB>(nf, j, id, g, lds, a.tw, a.P, a.Q, a.part, a.dc, red);
+  next_field_tail<T, N, TB>(nf, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, a.dc, red);
 }
 
 // widening copy of a spectrum into the FP64 state

@@ -131,6 +131,10 @@ Geo make_geo(int nx, int nz, double L) {
   g.kfac = 2.0 * PI / L;
   g.kfac2 = g.kfac * g.kfac;
   g.L3 = L * L * L;
+  g.pitch = g.nh;
+  g.aligned = 0;
+  g.pp = (long long)g.ny * g.nh;
+  g.ncp = g.nc;
   return g;
 }
 

@@ -20,10 +20,11 @@ struct SizeOps {
   int xlanes;      // row pairs per x workgroup
   int ytile;       // columns per y / z workgroup
   int ythreads;    // threads per y / z workgroup
-  void (*xr2c_d)(const double* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, hipStream_t s);
-  void (*xr2c)(const T* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, hipStream_t s);
-  void (*xc2r_d)(const cpx<T>* in, double* out, long long npairs, const cpx<T>* tw, hipStream_t s);
-  void (*xc2r)(const cpx<T>* in, T* out, long long npairs, const cpx<T>* tw, hipStream_t s);
+  void (*xr2c_d)(const double* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, const Geo& g,
+                 hipStream_t s);
+  void (*xr2c)(const T* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, const Geo& g, hipStream_t s);
+  void (*xc2r_d)(const cpx<T>* in, double* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
+  void (*xc2r)(const cpx<T>* in, T* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
   void (*xa)(const XaArgs<T>& a, bool withf, hipStream_t s);
   void (*xb)(const XbArgs<T>& a, hipStream_t s);
   void (*xe)(const XeArgs<T>& a, hipStream_t s);

@@ -15,17 +15,20 @@ struct Launch {
 
   static unsigned xgrid(long long npairs) { return unsigned((npairs + G - 1) / G); }
 
-  static void xr2c_d(const double* in, cpx<T>* out, long long np, const cpx<T>* tw, double scl, hipStream_t s) {
-    hipLaunchKernelGGL((k_xr2c<double, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, scl);
+  static Rows rows(const Geo& g) { return Rows{g.ny, g.pitch, g.pp}; }
+  static void xr2c_d(const double* in, cpx<T>* out, long long np, const cpx<T>* tw, double scl, const Geo& g,
+                     hipStream_t s) {
+    hipLaunchKernelGGL((k_xr2c<double, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, scl, rows(g));
   }
-  static void xr2c(const T* in, cpx<T>* out, long long np, const cpx<T>* tw, double scl, hipStream_t s) {
-    hipLaunchKernelGGL((k_xr2c<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, scl);
+  static void xr2c(const T* in, cpx<T>* out, long long np, const cpx<T>* tw, double scl, const Geo& g,
+                   hipStream_t s) {
+    hipLaunchKernelGGL((k_xr2c<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, scl, rows(g));
   }
-  static void xc2r_d(const cpx<T>* in, double* out, long long np, const cpx<T>* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_xc2r<T, double, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw);
+  static void xc2r_d(const cpx<T>* in, double* out, long long np, const cpx<T>* tw, const Geo& g, hipStream_t s) {
+    hipLaunchKernelGGL((k_xc2r<T, double, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
-  static void xc2r(const cpx<T>* in, T* out, long long np, const cpx<T>* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw);
+  static void xc2r(const cpx<T>* in, T* out, long long np, const cpx<T>* tw, const Geo& g, hipStream_t s) {
+    hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
   static void xa(const XaArgs<T>& a, bool withf, hipStream_t s) {
     if (withf)
@@ -40,17 +43,13 @@ struct Launch {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
   static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const long long batch = (long long)g.nz * g.nh;
-    dim3 grid(unsigned((batch + TB - 1) / TB), unsigned(narr));
+    dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr));
     if (sign < 0)
       hipLaunchKernelGGL((k_ypass<T, N, -1>), grid, dim3(YT), 0, s, b0, b1, g, tw);
     else
       hipLaunchKernelGGL((k_ypass<T, N, +1>), grid, dim3(YT), 0, s, b0, b1, g, tw);
   }
-  static unsigned zgrid(const Geo& g) {
-    const long long plane = (long long)g.ny * g.nh;
-    return unsigned((plane + TB - 1) / TB);
-  }
+  static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
     if (sign < 0)
       hipLaunchKernelGGL((k_zpass<T, N, -1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);

@@ -126,8 +126,9 @@ struct Solver final : SolverBase {
   // FP64 operations (state transforms, FP64 queries in mixed mode)
   const SizeOps<double>* oxd = nullptr;
   const SizeOps<double>* ozd = nullptr;
-  DBuf<cpx<double>> uh[2], pre, acc, S;   // state and combine accumulators: FP64
+  DBuf<cpx<double>> uh[2], S;            // state (FP64) and a transform scratch
   DBuf<cpx<T>> P, Q, Tt, W;              // transform work buffers
+  DBuf<cpx<T>> pre, Na, Nb;              // stage spectra kept for later stages
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
   DBuf<Scal> scal;
@@ -141,7 +142,11 @@ struct Solver final : SolverBase {
   int nxb = 0, nzb = 0;
   long long npairs = 0;
   int mmax = 0;
-  std::map<std::pair<int, double>, std::unique_ptr<DBuf<Coef>>> luts;
+  struct Lut {
+    DBuf<double2> c1;
+    DBuf<double4> c2;
+  };
+  std::map<std::pair<int, double>, std::unique_ptr<Lut>> luts;
 
   // pipeline state
   int cur = 0;
@@ -175,15 +180,26 @@ struct Solver final : SolverBase {
     oz = get_ops<T>(p.nz);
     oxd = get_ops<double>(p.nx);
     ozd = get_ops<double>(p.nz);
+    // padded, 128-byte aligned spectrum rows when every column tile width in
+    // use divides nx/2 (see Geo)
+    if ((p.nx / 2) % 16 == 0 && (p.nx / 2) % ox->ytile == 0 && (p.nx / 2) % oz->ytile == 0 &&
+        (p.nx / 2) % oxd->ytile == 0 && (p.nx / 2) % ozd->ytile == 0) {
+      g.pitch = p.nx / 2 + 16;
+      g.aligned = 1;
+      // one extra aligned row unit per plane keeps the z stride away from
+      // large powers of two (which concentrate z columns on few HBM channels)
+      g.pp = (long long)g.ny * g.pitch + 16;
+      g.ncp = (long long)g.nz * g.pp;
+    }
     if (shared) {
       st = shared;
     } else {
       VB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
       own_stream = true;
     }
-    const size_t nc = (size_t)g.nc, nr = (size_t)g.nr;
-    for (auto* b : {&uh[0], &uh[1], &pre, &acc, &S}) b->alloc(nc);
-    for (auto* b : {&P, &Q, &Tt}) b->alloc(nc);
+    const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
+    for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
+    for (auto* b : {&P, &Q, &Tt, &pre, &Na, &Nb}) b->alloc(nc);
     if (p.scheme == VB_EULER || p.shape_adjust) W.alloc(nc);
     dc.alloc(1);
     v.alloc(nr);
@@ -191,7 +207,7 @@ struct Solver final : SolverBase {
     phid.alloc(nr);
     npairs = (long long)g.nz * g.ny / 2;
     nxb = int((npairs + ox->xlanes - 1) / ox->xlanes);
-    nzb = int(((long long)g.ny * g.nh + oz->ytile - 1) / oz->ytile);
+    nzb = int(zgroups(g, oz->ytile));
     xpart.alloc((size_t)nxb * 3);
     zpart.alloc((size_t)nzb);
     scal.alloc(1);
@@ -233,20 +249,38 @@ struct Solver final : SolverBase {
     VB_HIP(hipMemcpy(b.p, h.data(), sizeof(cpx<X>) * (size_t)n, hipMemcpyHostToDevice));
   }
 
-  const Coef* lut(int sch, double h) {
+  const Lut* lut(int sch, double h) {
     auto key = std::make_pair(sch, h);
     auto it = luts.find(key);
-    if (it != luts.end()) return it->second->p;
+    if (it != luts.end()) return it->second.get();
     if (luts.size() > 64) {
       VB_HIP(hipStreamSynchronize(st));
       luts.clear();
     }
     std::vector<Coef> tab;
     build_lut(tab, mmax, sch, p.phi_lut_mode, h, g.kfac2, p.eps, p.kappa, p.D);
-    auto b = std::make_unique<DBuf<Coef>>();
-    b->alloc(tab.size());
-    VB_HIP(hipMemcpy(b->p, tab.data(), sizeof(Coef) * tab.size(), hipMemcpyHostToDevice));
-    const Coef* r = b->p;
+    // compact per-stage arrays: {e1, q} and {e2, A s, 2 B s, C s} (ETDRK2: {e, q1}, {c2})
+    std::vector<double2> h1(tab.size());
+    std::vector<double4> h2(tab.size());
+    for (size_t m = 0; m < tab.size(); ++m) {
+      h1[m].x = tab[m].c[0];
+      h1[m].y = tab[m].c[1];
+      if (sch == 2) {
+        h2[m].x = tab[m].c[2];
+        h2[m].y = h2[m].z = h2[m].w = 0.0;
+      } else {
+        h2[m].x = tab[m].c[2];
+        h2[m].y = tab[m].c[3];
+        h2[m].z = tab[m].c[4];
+        h2[m].w = tab[m].c[5];
+      }
+    }
+    auto b = std::make_unique<Lut>();
+    b->c1.alloc(tab.size());
+    b->c2.alloc(tab.size());
+    VB_HIP(hipMemcpy(b->c1.p, h1.data(), sizeof(double2) * h1.size(), hipMemcpyHostToDevice));
+    VB_HIP(hipMemcpy(b->c2.p, h2.data(), sizeof(double4) * h2.size(), hipMemcpyHostToDevice));
+    const Lut* r = b.get();
     luts.emplace(key, std::move(b));
     return r;
   }
@@ -342,6 +376,7 @@ struct Solver final : SolverBase {
     a.eps = p.eps;
     a.kappa = p.kappa;
     a.inv_n = 1.0 / double(g.nr);
+    a.rl = Rows{g.ny, g.pitch, g.pp};
     L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf, st); });
     spectral_lap(Tt.p);
     if (withf) spectral_lap(W.p);
@@ -357,7 +392,7 @@ struct Solver final : SolverBase {
     check_launch();
   }
 
-  void suffix(int sch, int stg, const Coef* lt, int out) {
+  void suffix(int sch, int stg, const Lut* lt, int out) {
     XbArgs<T> b;
     b.Tt = Tt.p;
     b.v = v.p;
@@ -370,22 +405,26 @@ struct Solver final : SolverBase {
     b.D = p.D;
     b.eta = p.eta;
     b.inv_n = 1.0 / double(g.nr);
+    b.rl = Rows{g.ny, g.pitch, g.pp};
     L("xb", [&] { ox->xb(b, st); });
     L("y_fwd", [&] { ox->ypass(Tt.p, nullptr, 1, -1, g, twx.p, st); });
     ZsArgs<T> z;
     z.U = Tt.p;
     z.uh = uh[cur].p;
     z.pre = pre.p;
-    z.acc = acc.p;
+    z.Na = Na.p;
+    z.Nb = Nb.p;
     z.uout = uh[out].p;
     z.P = P.p;
     z.Q = Q.p;
     z.part = zpart.p;
     z.dc = dc.p;
-    z.lut = lt;
+    z.c1 = lt->c1.p;
+    z.c2 = lt->c2.p;
     z.tw = twz.p;
     z.g = g;
-    L("zstage", [&] { oz->zstage(z, sch, stg, st); });
+    static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
+    L(names[stg], [&] { oz->zstage(z, sch, stg, st); });
     check_launch();
     prefixed = false;
     prefixed_f = false;
@@ -402,7 +441,7 @@ struct Solver final : SolverBase {
   // hold the prepared new state.
   int eval(double h) {
     const int sch = p.scheme == VB_ETDRK2 ? 2 : 4;
-    const Coef* lt = lut(sch, h);
+    const Lut* lt = lut(sch, h);
     const int out = 1 - cur;
     ensure_prefixed();
     for (int stg = 1; stg <= sch; ++stg) {
@@ -430,11 +469,12 @@ struct Solver final : SolverBase {
     e.t = t;
     e.inv_n = 1.0 / double(g.nr);
     e.shape = shape ? 1 : 0;
+    e.rl = Rows{g.ny, g.pitch, g.pp};
     L("xe", [&] { ox->xe(e, st); });
     L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
     L("widen", [&] {
-      hipLaunchKernelGGL(k_widen<T>, dim3(unsigned((g.nc + 255) / 256)), dim3(256), 0, st, W.p, uh[out].p, g.nc);
+      hipLaunchKernelGGL(k_widen<T>, dim3(unsigned((g.ncp + 255) / 256)), dim3(256), 0, st, W.p, uh[out].p, g.ncp);
     });
     check_launch();
     prefixed_f = false;   // W was consumed
@@ -634,7 +674,7 @@ struct Solver final : SolverBase {
   // Integrals of the state evaluated entirely in FP64 (mixed precision):
   // FP64 z, y and x transforms of the FP64 state, FP64 f and reductions.
   void integrals64() {
-    const size_t nc = (size_t)g.nc, nr = (size_t)g.nr;
+    const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
     if (!P64.p) {
       P64.alloc(nc);
       Q64.alloc(nc);
@@ -645,7 +685,7 @@ struct Solver final : SolverBase {
       scal64.alloc(1);
     }
     const int nxb64 = int((npairs + oxd->xlanes - 1) / oxd->xlanes);
-    const int nzb64 = int(((long long)g.ny * g.nh + ozd->ytile - 1) / ozd->ytile);
+    const int nzb64 = int(zgroups(g, ozd->ytile));
     if (xpart64.n < (size_t)nxb64 * 3) xpart64.alloc((size_t)nxb64 * 3);
     if (zpart64.n < (size_t)nzb64) zpart64.alloc((size_t)nzb64);
     L("zprep64", [&] { ozd->zprep(uh[cur].p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
@@ -663,6 +703,7 @@ struct Solver final : SolverBase {
     a.eps = p.eps;
     a.kappa = p.kappa;
     a.inv_n = 1.0 / double(g.nr);
+    a.rl = Rows{g.ny, g.pitch, g.pp};
     L("xa64", [&] { oxd->xa(a, false, st); });
     VB_HIP(hipMemcpyAsync(scal64.p, scal.p, sizeof(Scal), hipMemcpyDeviceToDevice, st));
     L("finalize", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
@@ -706,10 +747,10 @@ struct Solver final : SolverBase {
   }
 
   void get_phi(double* host) override {
-    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<double>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));
+    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<double>) * (size_t)g.ncp, hipMemcpyDeviceToDevice, st));
     L("z_inv", [&] { ozd->zpass(S.p, +1, g, twzd.p, st); });
     L("y_inv", [&] { oxd->ypass(S.p, nullptr, 1, +1, g, twxd.p, st); });
-    L("x_c2r", [&] { oxd->xc2r_d(S.p, phid.p, npairs, twxd.p, st); });
+    L("x_c2r", [&] { oxd->xc2r_d(S.p, phid.p, npairs, twxd.p, g, st); });
     check_launch();
     VB_HIP(hipMemcpyAsync(host, phid.p, sizeof(double) * (size_t)g.nr, hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
@@ -717,7 +758,7 @@ struct Solver final : SolverBase {
 
   void set_phi(const double* host) override {
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
-    L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), st); });
+    L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
     L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, st); });
     L("z_fwd", [&] { ozd->zpass(uh[cur].p, -1, g, twzd.p, st); });
     check_launch();
@@ -741,11 +782,11 @@ size_t solver_footprint(const vb_params& p) {
   const size_t nc = nh * (size_t)p.nx * (size_t)p.nz, nr = (size_t)p.nx * p.nx * p.nz;
   const bool mixed = p.precision == VB_FP32_MIXED;
   const size_t cs = mixed ? 8 : 16, rs = cs / 2;
-  const size_t work = 3 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
-  size_t bytes = 5 * nc * 16 + work * nc * cs + 2 * nr * rs + nr * 8;
+  const size_t work = 6 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
+  size_t bytes = 3 * nc * 16 + work * nc * cs + 2 * nr * rs + nr * 8;
   if (mixed) bytes += 3 * nc * 16 + 2 * nr * 8;   // FP64 query buffers
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
-  return bytes + 2 * mm * sizeof(Coef) + ((size_t)4 << 20);
+  return bytes + 2 * mm * 48 + ((size_t)4 << 20);
 }
 
 }  // namespace vb
