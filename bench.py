@@ -121,6 +121,19 @@ def cpu_baseline(n, seconds_cap=30.0):
                       "by the CPU oracle, %.1f s" % (n, dt)}
 
 
+def copy_calibration():
+    """A plain device copy of known size (256 MiB read + 256 MiB written), the
+    reference point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "repeats": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +143,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--latency-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver path runs on a single device and "
               "independent solves shard trivially; --gpus %d is not supported" % args.gpus, file=sys.stderr)
