@@ -1,0 +1,127 @@
+"""Summaries of rocprofv3 output for the solver's kernels.
+
+    python -m cudabubble_amd.rocprof_summary stats  KERNEL_STATS.csv  --cmd "..."
+    python -m cudabubble_amd.rocprof_summary traffic --fetch F.csv --write W.csv \
+        --cal-fetch CF.csv --cal-write CW.csv --n 256 --cmd "..." [--json-dir DIR --json-prec fp32]
+
+`stats` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown
+table keyed by the solver's own kernel names (the names `vb_profile` uses).
+`traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs into HBM bytes
+per launch, each counter scaled by what it reads for a plain device copy of
+known size (the `bench.py --copy-calibration` run), and optionally writes one
+`traffic_<kernel>_<n>_<prec>.json` per kernel for bench.py's roofline.
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import re
+import sys
+
+COPY_BYTES = 256 * 1024 * 1024     # bench.py --copy-calibration: bytes read and written per copy
+
+
+def solver_name(mangled):
+    """Maps a demangled rocprof kernel name to (solver kernel name, precision) or None."""
+    m = re.search(r"vb::k_(\w+)<(double|float)(?:, ?(-?\d+))?(?:, ?(-?\d+|true|false))?(?:, ?(-?\d+))?(?:, ?(-?\d+))?", mangled)
+    if not m:
+        if "k_finalize" in mangled:
+            return "finalize", None
+        if "k_fill_scal" in mangled:
+            return "fill_scal", None
+        return None
+    kind, prec = m.group(1), ("fp64" if m.group(2) == "double" else "fp32")
+    a = [m.group(i) for i in range(3, 7)]
+    if kind == "xa":
+        return ("xa_f" if a[1] == "true" else "xa"), prec
+    if kind == "ypass":
+        return ("y_fwd" if a[1] == "-1" else "y_inv"), prec
+    if kind == "zpass":
+        return ("z_fwd" if a[1] == "-1" else "z_inv"), prec
+    if kind == "zstage":
+        return "zstage%s_etdrk%s" % (a[2], a[1]) if a[1] != "4" else "zstage%s" % a[2], prec
+    if kind == "zlap":
+        return "z_lap", prec
+    return kind, prec
+
+
+def stats(path, cmd):
+    rows = list(csv.DictReader(open(path)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["# Kernel time, rocprofv3 --kernel-trace --stats", "",
+           "Command (repository root):", "", "    " + cmd, "",
+           "| kernel | precision | calls | avg us | total ms | % | rocprof name |",
+           "|---|---|---:|---:|---:|---:|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        sn = solver_name(r["Name"]) or ("-", None)
+        out.append("| %s | %s | %s | %.1f | %.3f | %.2f | `%s` |" % (
+            sn[0], sn[1] or "-", r["Calls"], float(r["AverageNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6,
+            100 * float(r["TotalDurationNs"]) / tot, r["Name"][:90]))
+    return "\n".join(out) + "\n"
+
+
+def _load(path):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return d
+
+
+def _copy_kib(cal):
+    vals = [v for k, vs in cal.items() if "copyBuffer" in k for v in vs]
+    if not vals:
+        raise SystemExit("no copy kernel in calibration file")
+    return sorted(vals)[len(vals) // 2]
+
+
+def traffic(fetch, write, cal_fetch, cal_write, n, cmds, json_dir=None, json_prec=None):
+    f, w = _load(fetch), _load(write)
+    ff = COPY_BYTES / 1024 / _copy_kib(_load(cal_fetch))
+    wf = COPY_BYTES / 1024 / _copy_kib(_load(cal_write))
+    rows = []
+    for k in f:
+        sn = solver_name(k)
+        if not sn or sn[1] is None:
+            continue
+        fk = sum(f[k]) / len(f[k]) * ff
+        wk = (sum(w[k]) / len(w[k]) * wf) if k in w else 0.0
+        rows.append((sn[0], sn[1], len(f[k]), fk * 1024, wk * 1024, k))
+    out = ["# HBM traffic per launch, rocprofv3 --pmc", "",
+           "FETCH_SIZE scaled by %.3f and WRITE_SIZE by %.3f: the factors that turn each counter's reading "
+           "for a plain %d MiB device copy into the bytes it moves." % (ff, wf, COPY_BYTES >> 20), "",
+           "Commands (repository root):", ""] + ["    " + c for c in cmds] + [
+           "", "| kernel | precision | launches | read MB | written MB | total MB |", "|---|---|---:|---:|---:|---:|"]
+    for name, prec, cnt, rb, wb, k in sorted(rows, key=lambda r: -(r[3] + r[4])):
+        out.append("| %s | %s | %d | %.1f | %.1f | %.1f |" % (name, prec, cnt, rb / 1e6, wb / 1e6, (rb + wb) / 1e6))
+        if json_dir and (json_prec is None or prec == json_prec):
+            with open(os.path.join(json_dir, "traffic_%s_%d_%s.json" % (name, n, prec)), "w") as fh:
+                json.dump({"kernel": name, "n": n, "precision": prec, "launches": cnt,
+                           "read_bytes_per_launch": rb, "write_bytes_per_launch": wb,
+                           "bytes_per_launch": rb + wb, "fetch_factor": ff, "write_factor": wf,
+                           "rocprof_name": k}, fh, indent=1)
+    return "\n".join(out) + "\n"
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--cmd", required=True)
+    t = sub.add_parser("traffic")
+    for a in ("--fetch", "--write", "--cal-fetch", "--cal-write"):
+        t.add_argument(a, required=True)
+    t.add_argument("--n", type=int, required=True)
+    t.add_argument("--cmd", action="append", required=True)
+    t.add_argument("--json-dir")
+    t.add_argument("--json-prec", help="write JSON only for this precision (fp32 / fp64)")
+    a = ap.parse_args(argv)
+    if a.what == "stats":
+        sys.stdout.write(stats(a.csv, a.cmd))
+    else:
+        sys.stdout.write(traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.n, a.cmd, a.json_dir, a.json_prec))
+
+
+if __name__ == "__main__":
+    main()
