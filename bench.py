@@ -121,6 +121,65 @@ def cpu_baseline(n, seconds=10.0):
                       "by the CPU oracle, %.1f s timed" % (evals, n, dt)}
 
 
+BATCH_GROUPS = ((32, 16), (64, 16), (128, 12), (192, 4))    # (n, solves)
+
+
+def bending_energy(phi, eps, L):
+    """E_b = (1/2eps) int f^2, f = -eps lap phi + (phi^2-1) phi / eps, by numpy FFT
+    (host-side set-up of the batched form only)."""
+    n = phi.shape[-1]
+    k = np.fft.fftfreq(n, 1.0 / n) * (2 * np.pi / L)
+    kz = np.fft.fftfreq(phi.shape[0], 1.0 / phi.shape[0]) * (2 * np.pi / L)
+    k2 = kz[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
+    lap = np.real(np.fft.ifftn(-k2 * np.fft.fftn(phi)))
+    f = -eps * lap + (phi * phi - 1) * phi / eps
+    return float(np.mean(f * f)) * L ** 3 / (2 * eps)
+
+
+def batched(vb, steps, seed=20260004):
+    """The batched form: 48 FP32-mixed ETDRK4 solves of uneven size resident
+    together on the GPU (16 x 32^3, 16 x 64^3, 12 x 128^3, 4 x 192^3).  Solve j
+    is an ellipsoid, torus or biconcave disk (in turn) drawn by vb_shape from
+    seed + j; eps in {1.5h, 2h, 3h} and M1, M2 log-uniform in [1e3, 1e5] come
+    from numpy's PCG64(seed); fixed dt with the penalty drift (two evaluations
+    per step).  The penalties enter N explicitly, so their stiffness bounds dt:
+    the volume mode relaxes at rate M1 L^3 / 4 and the area mode at about
+    M2 (9/8) 2 eps E_b, and the reference scheme diverges once dt times their
+    sum exceeds about 1 (torus and disk shapes at M2 ~ 1e5 have E_b ~ 1e2-1e3).
+    Each solve takes dt = min(1e-7, 0.5 / that sum) from its initial field."""
+    rng = np.random.default_rng(seed)
+    ps, phis = [], []
+    j = 0
+    for n, count in BATCH_GROUPS:
+        for _ in range(count):
+            kind = (vb.SHAPE_ELLIPSOID, vb.SHAPE_TORUS, vb.SHAPE_DISK)[j % 3]
+            eps = (1.5, 2.0, 3.0)[int(rng.integers(3))] * (2 * np.pi) / n
+            M1, M2 = 10.0 ** rng.uniform(3, 5, size=2)
+            phi, _ = vb.shape(kind, seed + j, n, n, 1.0, eps)
+            L = 2 * np.pi
+            stiff = M1 * L ** 3 / 4 + M2 * 9.0 / 8.0 * 2 * eps * bending_energy(phi, eps, L)
+            dt = min(1e-7, 0.5 / stiff)
+            ps.append(vb.params_default(nx=n, nz=n, eps=eps, M1=M1, M2=M2, alpha=0.0, beta=0.0,
+                                        time_step=dt, fixed_time_step=dt, precision=vb.FP32_MIXED,
+                                        phi_lut_mode=vb.LUT_ACCURATE, scheme=vb.ETDRK4))
+            phis.append(phi)
+            j += 1
+    b = vb.Batch(ps, phis)
+    b.step(2)     # start-up: alpha/beta, tables, graph capture
+    sync()
+    t0 = time.perf_counter()
+    reps = b.step(steps)
+    sync()
+    dt = time.perf_counter() - t0
+    gp_evals = sum(p.nx * p.nx * p.nz * r["evaluations"] for p, r in zip(ps, reps))
+    b.close()
+    return {"value": gp_evals / dt, "unit": "grid-point-steps/s", "solves": len(ps), "steps": steps,
+            "not_ok": [i for i, r in enumerate(reps) if r["status"] != 0],
+            "dt_range": [min(p.fixed_time_step for p in ps), max(p.fixed_time_step for p in ps)],
+            "seconds": dt, "grid_points": int(sum(p.nx * p.nx * p.nz for p in ps)),
+            "groups": ["%d x %d^3" % (c, n) for n, c in BATCH_GROUPS]}
+
+
 def copy_calibration():
     """A plain device copy of known size (256 MiB read + 256 MiB written), the
     reference point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -143,6 +202,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--latency-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=300,
+                    help="steps of the 48-solve batched form (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -215,6 +276,11 @@ def main():
     out_extra["kernels_fp32"] = [{"name": nm, "launches": c, "ms": round(m, 3),
                                   "GBps": (bb / (m / c * 1e-3) / 1e9) if bb else None}
                                  for m, nm, c, bb in prof32]
+    if args.batch_steps > 0:
+        bt = batched(vb, args.batch_steps)
+        if "fp64" in out_extra:
+            bt["ratio_vs_fp64_single_%d" % n] = bt["value"] / out_extra["fp64"]["value"]
+        out_extra["batched"] = bt
     cpu = None if args.no_cpu else cpu_baseline(n)
 
     line = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,

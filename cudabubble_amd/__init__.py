@@ -43,16 +43,32 @@ class Params(C.Structure):
                 ("fixed_time_step", C.c_double), ("dt_cap", C.c_double),
                 ("dt_double_every", C.c_int), ("penalty_drift", C.c_int),
                 ("phi_lut_mode", C.c_int), ("shape_adjust", C.c_int), ("tmax", C.c_double),
-                ("device", C.c_int), ("pad", C.c_int)]
+                ("device", C.c_int), ("graphs", C.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class StepReport(C.Structure):
     _fields_ = [("steps", C.c_int), ("evaluations", C.c_int), ("rejections", C.c_int),
                 ("status", C.c_int), ("time_step", C.c_double), ("time_current", C.c_double),
                 ("energy", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_active", C.c_int), ("streams", C.c_int), ("keep_phi", C.c_int),
+                ("memory_budget", C.c_size_t), ("converge_window", C.c_int), ("max_steps", C.c_int),
+                ("converge_tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("steps", C.c_int), ("evaluations", C.c_int),
+                ("rejections", C.c_int), ("E", C.c_double), ("V", C.c_double), ("A", C.c_double),
+                ("Eb", C.c_double), ("time_step", C.c_double), ("time_current", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -107,9 +123,19 @@ def lib():
     L.vb_lut.argtypes = [I, I, D, D, D, D, D, I, P]
     L.vb_batch_create.argtypes = [I, C.POINTER(Params), C.POINTER(P), C.POINTER(P)]
     L.vb_batch_step.argtypes = [P, I]
+    L.vb_batch_report.argtypes = [P, I, C.POINTER(StepReport)]
     L.vb_batch_query.argtypes = [P, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(D)]
     L.vb_batch_get_phi.argtypes = [P, I, P]
     L.vb_batch_destroy.argtypes = [P]
+    L.vb_server_config_default.argtypes = [C.POINTER(ServerConfig)]
+    L.vb_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
+    L.vb_server_submit.argtypes = [P, C.c_longlong, C.POINTER(Params), P]
+    L.vb_server_poll.argtypes = [P, C.POINTER(Result), I, C.POINTER(I)]
+    L.vb_server_pending.argtypes = [P, C.POINTER(I), C.POINTER(I), C.POINTER(S)]
+    L.vb_server_take_phi.argtypes = [P, C.c_longlong, P]
+    L.vb_server_clock.restype = D
+    L.vb_server_clock.argtypes = [P]
+    L.vb_server_destroy.argtypes = [P]
     for name, nargs in [("vb_calc_r2c", 2), ("vb_calc_c2r", 2), ("vb_calc_laplace", 1),
                         ("vb_calc_lap_spec", 2), ("vb_calc_lap_real", 2),
                         ("vb_calc_fftgradient", 4), ("vb_calc_multi", 3), ("vb_calc_sync", 0)]:
@@ -269,7 +295,8 @@ class Solver:
 
 
 class Batch:
-    """Many independent solves sharing one device stream."""
+    """Many independent solves of any supported sizes on one GPU (vb_batch_*):
+    resident per-solve state, a few streams, graph replays of fixed-dt steps."""
 
     def __init__(self, params_list, phi_list):
         n = len(params_list)
@@ -282,7 +309,14 @@ class Batch:
         self.h = h
 
     def step(self, n=1):
+        """n driver iterations of every solve; returns their step reports."""
         _check(lib().vb_batch_step(self.h, int(n)), "vb_batch_step")
+        return [self.report(i) for i in range(len(self.params))]
+
+    def report(self, i):
+        r = StepReport()
+        _check(lib().vb_batch_report(self.h, i, C.byref(r)), "vb_batch_report")
+        return r.as_dict()
 
     def query(self, i):
         E, V, A, Eb = C.c_double(), C.c_double(), C.c_double(), C.c_double()
@@ -298,6 +332,70 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().vb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop on one GPU (vb_server_*): submit solves at any time, poll
+    finished results.  Requests use adaptive dt and an ETD scheme."""
+
+    def __init__(self, max_active=16, streams=4, keep_phi=False, memory_budget=0, converge_window=50,
+                 converge_tol=1e-6, max_steps=0, device=0):
+        c = ServerConfig()
+        lib().vb_server_config_default(C.byref(c))
+        c.device, c.max_active, c.streams, c.keep_phi = device, max_active, streams, int(keep_phi)
+        c.memory_budget, c.converge_window, c.max_steps, c.converge_tol = memory_budget, converge_window, \
+            max_steps, converge_tol
+        h = C.c_void_p()
+        _check(lib().vb_server_create(C.byref(c), C.byref(h)), "vb_server_create")
+        self.h, self.shapes = h, {}
+
+    def submit(self, rid, params, phi0):
+        phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+        if phi0.shape != (params.nz, params.nx, params.nx):
+            raise ValueError("phi0 shape %s != (nz, ny, nx)" % (phi0.shape,))
+        _check(lib().vb_server_submit(self.h, int(rid), C.byref(params), _np_ptr(phi0)), "vb_server_submit")
+        self.shapes[int(rid)] = phi0.shape
+
+    def poll(self, cap=256):
+        out = (Result * cap)()
+        n = C.c_int()
+        _check(lib().vb_server_poll(self.h, out, cap, C.byref(n)), "vb_server_poll")
+        return [out[i].as_dict() for i in range(n.value)]
+
+    def pending(self):
+        q, r, b = C.c_int(), C.c_int(), C.c_size_t()
+        _check(lib().vb_server_pending(self.h, C.byref(q), C.byref(r), C.byref(b)), "vb_server_pending")
+        return {"queued": q.value, "running": r.value, "bytes": b.value}
+
+    def take_phi(self, rid):
+        out = np.zeros(self.shapes[int(rid)])
+        _check(lib().vb_server_take_phi(self.h, int(rid), _np_ptr(out)), "vb_server_take_phi")
+        return out
+
+    def clock(self):
+        return lib().vb_server_clock(self.h)
+
+    def drain(self, poll_s=0.002):
+        """Waits until nothing is queued or running; returns the results."""
+        import time
+        res = []
+        while True:
+            res += self.poll()
+            st = self.pending()
+            if st["queued"] == 0 and st["running"] == 0:
+                return res + self.poll()
+            time.sleep(poll_s)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vb_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -148,6 +148,7 @@ void vb_params_default(vb_params* p) {
   p->shape_adjust = 0;
   p->tmax = 4.0;
   p->device = 0;
+  p->graphs = 1;
 }
 
 int vb_params_from_ini(const char* path, vb_params* out) {

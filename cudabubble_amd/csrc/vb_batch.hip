@@ -1,9 +1,15 @@
 // vb_batch.hip -- many independent vesicle solves on one GPU.
 //
-// Every solve keeps its own resident state and work buffers in HBM; all solves
-// of a batch launch on one shared stream, in size-sorted order, so consecutive
-// launches of small grids run back to back on the device without host
-// synchronisation.
+// Every solve keeps its state, work buffers and coefficient tables resident in
+// HBM for the life of the batch.  Solves are spread over a few streams by
+// longest-processing-time assignment (grid points per stream balanced), and a
+// batch step issues the solves' fixed-dt iterations round-robin, two at a time,
+// each pair one replay of the solve's captured graph.  Small grids, whose
+// kernels cannot fill 256 CUs, then run concurrently with the large ones, and
+// the host issues one graph launch per solve and pair of iterations instead
+// of about sixty kernel launches per iteration.  Solves with adaptive dt need
+// a host decision every iteration; they step after the fixed-dt work has been
+// issued and overlap with it.
 #include <algorithm>
 #include <memory>
 #include <numeric>
@@ -12,15 +18,20 @@
 #include "vb_host.hpp"
 #include "vb_solver.hpp"
 
+namespace {
+constexpr int kBatchStreams = 4;   // one per hardware queue of the process
+}
+
 struct vb_batch {
   std::vector<std::unique_ptr<vb::SolverBase>> solves;
-  std::vector<int> order;
-  hipStream_t st = nullptr;
+  std::vector<int> order;                 // decreasing grid size
+  std::vector<hipStream_t> streams;
+  std::vector<vb_step_report> reports;    // of the last vb_batch_step
   ~vb_batch() {
     solves.clear();
-    if (st) {
-      (void)hipStreamSynchronize(st);
-      (void)hipStreamDestroy(st);
+    for (auto s : streams) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
     }
   }
 };
@@ -37,6 +48,8 @@ int bguard(F&& f) {
     return VB_ERR_OOM;
   }
 }
+
+long long points(const vb_params& p) { return (long long)p.nx * p.nx * p.nz; }
 }  // namespace
 
 extern "C" {
@@ -46,23 +59,38 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
   *out = nullptr;
   for (int i = 0; i < n; ++i) {
     const vb_params& p = params[i];
-    if (p.nx <= 0 || p.nz <= 0 || (p.nx % 2) || !(p.eps > 0) || !(p.len > 0) || !phi0[i]) return VB_ERR_PARAM;
+    if (p.nx <= 0 || p.nz <= 0 || !(p.eps > 0) || !(p.len > 0) || !phi0[i]) return VB_ERR_PARAM;
+    if (p.nx % 2) return VB_ERR_ODD_NX;
     if (!vb::size_supported(p.nx) || !vb::size_supported(p.nz)) return VB_ERR_PARAM;
+    if (p.device != params[0].device) return VB_ERR_PARAM;
   }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= params[0].device) return VB_ERR_NO_DEVICE;
   return bguard([&] {
     auto b = std::make_unique<vb_batch>();
     VB_HIP(hipSetDevice(params[0].device));
-    VB_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
-    for (int i = 0; i < n; ++i) {
-      b->solves.push_back(vb::make_solver(params[i], b->st));
-      b->solves.back()->set_phi(phi0[i]);
+    const int ns = std::min(n, kBatchStreams);
+    for (int k = 0; k < ns; ++k) {
+      hipStream_t s;
+      VB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      b->streams.push_back(s);
     }
     b->order.resize(n);
     std::iota(b->order.begin(), b->order.end(), 0);
-    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
-      return (long long)params[a].nx * params[a].nx * params[a].nz >
-             (long long)params[c].nx * params[c].nx * params[c].nz;
-    });
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int a, int c) { return points(params[a]) > points(params[c]); });
+    std::vector<int> stream_of(n);
+    std::vector<long long> load(ns, 0);
+    for (int i : b->order) {
+      const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
+      stream_of[i] = k;
+      load[k] += points(params[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+      b->solves.push_back(vb::make_solver(params[i], b->streams[stream_of[i]]));
+      b->solves.back()->set_phi(phi0[i]);
+    }
+    b->reports.assign(n, vb_step_report{});
     *out = b.release();
     return (int)VB_OK;
   });
@@ -71,13 +99,37 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
 int vb_batch_step(vb_batch* b, int n_steps) {
   if (!b || n_steps < 0) return VB_ERR_PARAM;
   return bguard([&] {
-    for (int i : b->order) {
-      vb_step_report r{};
-      b->solves[(size_t)i]->step(n_steps, &r);
+    for (auto& r : b->reports) {
+      r = vb_step_report{};
+      r.energy = NAN;
     }
-    VB_HIP(hipStreamSynchronize(b->st));
+    // fixed-dt solves: iterations issued round-robin, a graph replay at a time
+    constexpr int chunk = 2;
+    for (int done = 0; done < n_steps; done += chunk) {
+      const int k = std::min(chunk, n_steps - done);
+      for (int i : b->order) {
+        auto& s = b->solves[(size_t)i];
+        if (s->fixed_dt() && b->reports[(size_t)i].status == VB_OK) s->launch_steps(k, &b->reports[(size_t)i]);
+      }
+    }
+    // adaptive solves decide on the host every iteration
+    for (int i : b->order) {
+      auto& s = b->solves[(size_t)i];
+      if (!s->fixed_dt()) s->step(n_steps, &b->reports[(size_t)i]);
+    }
+    for (int i : b->order) {
+      auto& s = b->solves[(size_t)i];
+      if (s->fixed_dt()) s->finish_steps(&b->reports[(size_t)i]);
+    }
+    for (auto s : b->streams) VB_HIP(hipStreamSynchronize(s));
     return (int)VB_OK;
   });
+}
+
+int vb_batch_report(vb_batch* b, int i, vb_step_report* rep) {
+  if (!b || i < 0 || i >= (int)b->solves.size() || !rep) return VB_ERR_PARAM;
+  *rep = b->reports[(size_t)i];
+  return VB_OK;
 }
 
 int vb_batch_query(vb_batch* b, int i, double* E, double* V, double* A, double* Eb) {

@@ -1,0 +1,324 @@
+// vb_server.hip -- a request loop for vesicle solves on one GPU.
+//
+// Clients submit solves (parameters + initial field) at any time; one worker
+// thread owns the device.  It admits pending requests in arrival order while
+// the resident footprint fits the memory budget and a slot is free, keeps
+// every admitted solve's state, work buffers and coefficient tables resident
+// in HBM, and interleaves the solves' adaptive iterations: every running solve
+// has one attempt in flight on one of a few streams, and whichever attempt
+// finishes first is decided on the host and its next attempt issued.  A solve
+// leaves when its energy has dropped by less than a relative tolerance over a
+// window of accepted steps, when t reaches tmax, at a driver exit, or at a
+// step cap.  Requests are independent: each computes exactly what vb_step
+// computes for it alone, whatever else is running.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "vb_host.hpp"
+#include "vb_solver.hpp"
+
+namespace {
+
+struct Request {
+  long long id;
+  vb_params p;
+  std::vector<double> phi0;
+  double t_submit;
+};
+
+struct Running {
+  Request req;
+  std::unique_ptr<vb::SolverBase> s;
+  int stream;
+  size_t bytes;
+  double t_admit;
+  vb_step_report rep{};
+  std::deque<double> hist;   // energies of the last window+1 accepted states
+};
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct vb_server {
+  vb_server_config cfg{};
+  std::chrono::steady_clock::time_point t0;
+  std::vector<hipStream_t> streams;
+  std::vector<long long> stream_load;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Request> pending;
+  std::vector<vb_result> done;
+  std::map<long long, std::vector<double>> phis;
+  std::vector<Running> running;   // worker thread only
+  size_t used = 0;
+  int nrun = 0;
+  bool stop = false;
+  int worker_error = VB_OK;
+  std::thread worker;
+
+  ~vb_server() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    if (worker.joinable()) worker.join();
+    running.clear();
+    for (auto s : streams) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+
+  void finish(Running& r) {
+    vb_result res{};
+    res.id = r.req.id;
+    res.status = r.rep.status;
+    res.steps = r.rep.steps;
+    res.evaluations = r.rep.evaluations;
+    res.rejections = r.rep.rejections;
+    r.s->query(&res.E, &res.V, &res.A, &res.Eb);
+    r.s->state(nullptr, nullptr, &res.time_step, &res.time_current);
+    res.t_submit = r.req.t_submit;
+    res.t_admit = r.t_admit;
+    std::vector<double> phi;
+    if (cfg.keep_phi) {
+      phi.resize((size_t)r.req.p.nx * r.req.p.nx * r.req.p.nz);
+      r.s->get_phi(phi.data());
+    }
+    res.t_done = now_s(t0);
+    stream_load[(size_t)r.stream] -= (long long)r.req.p.nx * r.req.p.nx * r.req.p.nz;
+    r.s.reset();
+    std::lock_guard<std::mutex> g(mu);
+    used -= r.bytes;
+    nrun--;
+    if (cfg.keep_phi) phis[res.id] = std::move(phi);
+    done.push_back(res);
+  }
+
+  // true when the solve leaves the server after this accepted iteration
+  bool converged(Running& r) {
+    if (r.rep.status != VB_OK) return true;
+    r.hist.push_back(r.rep.energy);
+    if ((int)r.hist.size() > cfg.converge_window + 1) r.hist.pop_front();
+    if ((int)r.hist.size() == cfg.converge_window + 1) {
+      const double e0 = r.hist.front(), e1 = r.hist.back();
+      if (fabs(e0 - e1) <= cfg.converge_tol * fabs(e0)) return true;
+    }
+    double t;
+    r.s->state(nullptr, nullptr, nullptr, &t);
+    if (r.req.p.tmax > 0 && t >= r.req.p.tmax) return true;
+    if (cfg.max_steps > 0 && r.rep.steps >= cfg.max_steps) return true;
+    return false;
+  }
+
+  void admit() {
+    for (;;) {
+      Request q;
+      size_t bytes;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        if (pending.empty() || (int)running.size() >= cfg.max_active) return;
+        bytes = vb::solver_footprint(pending.front().p);
+        if (used + bytes > cfg.memory_budget && !running.empty()) return;   // wait for memory
+        q = std::move(pending.front());
+        pending.pop_front();
+        used += bytes;
+        nrun++;
+      }
+      const int k = int(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
+      Running r;
+      r.stream = k;
+      r.bytes = bytes;
+      r.t_admit = now_s(t0);
+      r.rep.energy = NAN;
+      int fail = VB_OK;
+      try {
+        r.s = vb::make_solver(q.p, streams[(size_t)k]);
+        r.s->set_phi(q.phi0.data());
+      } catch (const vb::HipError& e) {
+        fail = e.err == hipErrorOutOfMemory ? VB_ERR_OOM : VB_ERR_HIP;
+      } catch (const std::bad_alloc&) {
+        fail = VB_ERR_OOM;
+      }
+      if (fail != VB_OK) {
+        r.s.reset();
+        vb_result res{};
+        res.id = q.id;
+        res.status = fail;
+        res.t_submit = q.t_submit;
+        res.t_admit = r.t_admit;
+        res.t_done = now_s(t0);
+        std::lock_guard<std::mutex> g(mu);
+        used -= bytes;
+        nrun--;
+        done.push_back(res);
+        continue;
+      }
+      stream_load[(size_t)k] += (long long)q.p.nx * q.p.nx * q.p.nz;
+      q.phi0.clear();
+      q.phi0.shrink_to_fit();
+      r.req = std::move(q);
+      running.push_back(std::move(r));
+      running.back().s->adaptive_issue(running.back().rep);
+    }
+  }
+
+  void loop() {
+    try {
+      for (;;) {
+        admit();
+        if (running.empty()) {
+          std::unique_lock<std::mutex> g(mu);
+          if (stop && pending.empty()) return;
+          cv.wait(g, [&] { return stop || !pending.empty(); });
+          continue;
+        }
+        bool progressed = false;
+        for (size_t i = 0; i < running.size();) {
+          Running& r = running[i];
+          if (!r.s->attempt_ready()) {
+            ++i;
+            continue;
+          }
+          progressed = true;
+          const int ended = r.s->adaptive_complete(r.rep);
+          if (ended && converged(r)) {
+            finish(r);
+            running.erase(running.begin() + (long)i);
+            continue;
+          }
+          r.s->adaptive_issue(r.rep);
+          ++i;
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+      }
+    } catch (const vb::HipError& e) {
+      vb::set_last_error(std::string(hipGetErrorString(e.err)) + " in " + e.where);
+      std::lock_guard<std::mutex> g(mu);
+      worker_error = e.err == hipErrorOutOfMemory ? VB_ERR_OOM : VB_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+      std::lock_guard<std::mutex> g(mu);
+      worker_error = VB_ERR_OOM;
+    }
+  }
+};
+
+extern "C" {
+
+void vb_server_config_default(vb_server_config* c) {
+  if (!c) return;
+  *c = vb_server_config{};
+  c->device = 0;
+  c->max_active = 16;
+  c->streams = 4;
+  c->keep_phi = 0;
+  c->memory_budget = 0;
+  c->converge_window = 50;
+  c->converge_tol = 1e-6;
+  c->max_steps = 0;
+}
+
+int vb_server_create(const vb_server_config* cfg, vb_server** out) {
+  if (!cfg || !out || cfg->max_active <= 0 || cfg->streams <= 0 || cfg->converge_window <= 0) return VB_ERR_PARAM;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) return VB_ERR_NO_DEVICE;
+  try {
+    auto s = std::make_unique<vb_server>();
+    s->cfg = *cfg;
+    s->t0 = std::chrono::steady_clock::now();
+    VB_HIP(hipSetDevice(cfg->device));
+    if (s->cfg.memory_budget == 0) {
+      size_t fr = 0, tot = 0;
+      VB_HIP(hipMemGetInfo(&fr, &tot));
+      s->cfg.memory_budget = fr / 10 * 9;
+    }
+    for (int k = 0; k < cfg->streams; ++k) {
+      hipStream_t st;
+      VB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      s->streams.push_back(st);
+      s->stream_load.push_back(0);
+    }
+    vb_server* raw = s.get();
+    s->worker = std::thread([raw] {
+      (void)hipSetDevice(raw->cfg.device);
+      raw->loop();
+    });
+    *out = s.release();
+    return VB_OK;
+  } catch (const vb::HipError& e) {
+    vb::set_last_error(std::string(hipGetErrorString(e.err)) + " in " + e.where);
+    return VB_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return VB_ERR_OOM;
+  }
+}
+
+int vb_server_submit(vb_server* s, long long id, const vb_params* p, const double* phi0) {
+  if (!s || !p || !phi0) return VB_ERR_PARAM;
+  if (p->nx <= 0 || p->nz <= 0 || !(p->eps > 0) || !(p->len > 0)) return VB_ERR_PARAM;
+  if (p->nx % 2) return VB_ERR_ODD_NX;
+  if (!vb::size_supported(p->nx) || !vb::size_supported(p->nz)) return VB_ERR_PARAM;
+  if (p->fixed_time_step > 0 || p->scheme == VB_EULER) return VB_ERR_PARAM;   // adaptive ETD only
+  if (p->device != s->cfg.device) return VB_ERR_PARAM;
+  Request q;
+  q.id = id;
+  q.p = *p;
+  q.phi0.assign(phi0, phi0 + (size_t)p->nx * p->nx * p->nz);
+  q.t_submit = now_s(s->t0);
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->worker_error) return s->worker_error;
+    if (vb::solver_footprint(*p) > s->cfg.memory_budget) return VB_ERR_OOM;
+    s->pending.push_back(std::move(q));
+  }
+  s->cv.notify_all();
+  return VB_OK;
+}
+
+int vb_server_poll(vb_server* s, vb_result* out, int cap, int* n_out) {
+  if (!s || !n_out || (cap > 0 && !out)) return VB_ERR_PARAM;
+  std::lock_guard<std::mutex> g(s->mu);
+  const int n = std::min(cap, (int)s->done.size());
+  for (int i = 0; i < n; ++i) out[i] = s->done[(size_t)i];
+  s->done.erase(s->done.begin(), s->done.begin() + n);
+  *n_out = n;
+  return s->worker_error;
+}
+
+int vb_server_pending(vb_server* s, int* queued, int* running, size_t* bytes_in_use) {
+  if (!s) return VB_ERR_PARAM;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (queued) *queued = (int)s->pending.size();
+  if (running) *running = s->nrun;
+  if (bytes_in_use) *bytes_in_use = s->used;
+  return s->worker_error;
+}
+
+int vb_server_take_phi(vb_server* s, long long id, double* host_out) {
+  if (!s || !host_out) return VB_ERR_PARAM;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->phis.find(id);
+  if (it == s->phis.end()) return VB_ERR_PARAM;
+  std::copy(it->second.begin(), it->second.end(), host_out);
+  s->phis.erase(it);
+  return VB_OK;
+}
+
+double vb_server_clock(vb_server* s) { return s ? now_s(s->t0) : 0.0; }
+
+void vb_server_destroy(vb_server* s) { delete s; }
+
+}  // extern "C"

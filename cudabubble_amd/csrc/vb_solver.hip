@@ -166,6 +166,31 @@ struct Solver final : SolverBase {
   std::vector<hipEvent_t> evpool;
   std::map<std::string, std::pair<int, double>> ktimes;
 
+  // Fixed-dt driver iterations captured as HIP graphs.  A graph records the
+  // launches two iterations issue from one host-side pipeline state (the
+  // state index cur, the prepared/prefixed flags and dt) and is replayed only
+  // from that same state; afterwards the host state is set to what the two
+  // iterations leave behind, exactly as if they had run eagerly.
+  struct PipeState {
+    int cur;
+    bool prepped, prefixed, prefixed_f;
+    double dt;
+    bool operator==(const PipeState& o) const {
+      return cur == o.cur && prepped == o.prepped && prefixed == o.prefixed && prefixed_f == o.prefixed_f &&
+             dt == o.dt;
+    }
+  };
+  struct StepGraph {
+    hipGraphExec_t x = nullptr;
+    PipeState pre{}, post{};
+    int evals = 0;
+  };
+  static constexpr int kGraphIters = 2;
+  StepGraph graphs[4];
+  int graph_next = 0;
+  PipeState fixed_post{};        // state after the last fixed iteration
+  bool have_fixed_post = false;
+
   Solver(const vb_params& pp, hipStream_t shared) {
     p = pp;
     g = make_geo(p.nx, p.nz, p.len * (2.0 * 3.1415926535897932384626433832795));
@@ -230,6 +255,9 @@ struct Solver final : SolverBase {
 
   ~Solver() override {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& G : graphs)
+      if (G.x) (void)hipGraphExecDestroy(G.x);
+    if (att_ev) (void)hipEventDestroy(att_ev);
     flush_prof();
     for (auto e : evpool) (void)hipEventDestroy(e);
     if (hs) (void)hipHostFree(hs);
@@ -254,7 +282,13 @@ struct Solver final : SolverBase {
     auto it = luts.find(key);
     if (it != luts.end()) return it->second.get();
     if (luts.size() > 64) {
+      // captured graphs hold table addresses: drop them with the tables
       VB_HIP(hipStreamSynchronize(st));
+      for (auto& G : graphs)
+        if (G.x) {
+          VB_HIP(hipGraphExecDestroy(G.x));
+          G.x = nullptr;
+        }
       luts.clear();
     }
     std::vector<Coef> tab;
@@ -481,6 +515,216 @@ struct Solver final : SolverBase {
     return out;
   }
 
+  bool drift_mode() const { return (p.M1 != 0 || p.M2 != 0) && p.penalty_drift; }
+
+  // One fixed-dt iteration of the run_static_fEIF loop, launches only: the
+  // penalty-drift double step (belastic.cpp:670-691) or a plain step
+  // (belastic.cpp:692-693).  Returns the number of evaluations.
+  int fixed_body() {
+    if (drift_mode()) {
+      ensure_prefixed();
+      int out = eval(time_step);
+      const int keep = cur;
+      cur = out;
+      prefix(FIN_DRIFT, false);
+      cur = keep;
+      prepped = prefixed = prefixed_f = false;
+      ensure_prefixed();
+      out = eval(time_step);
+      cur = out;
+      prefix(FIN_DRIFT, false);
+      return 2;
+    }
+    ensure_prefixed();
+    cur = eval(time_step);
+    return 1;
+  }
+
+  PipeState pipe_state() const { return PipeState{cur, prepped, prefixed, prefixed_f, time_step}; }
+  void set_pipe_state(const PipeState& q) {
+    cur = q.cur;
+    prepped = q.prepped;
+    prefixed = q.prefixed;
+    prefixed_f = q.prefixed_f;
+  }
+
+  // Records kGraphIters fixed iterations from the current pipeline state.
+  // Nothing runs during the capture; the host state is put back afterwards.
+  StepGraph& capture_iters() {
+    StepGraph& G = graphs[graph_next];
+    graph_next = (graph_next + 1) % 4;
+    if (G.x) {
+      VB_HIP(hipGraphExecDestroy(G.x));
+      G.x = nullptr;
+    }
+    (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // table upload happens outside the capture
+    G.pre = pipe_state();
+    G.evals = 0;
+    VB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t gr = nullptr;
+    try {
+      for (int i = 0; i < kGraphIters; ++i) G.evals += fixed_body();
+    } catch (...) {
+      (void)hipStreamEndCapture(st, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      set_pipe_state(G.pre);
+      throw;
+    }
+    G.post = pipe_state();
+    set_pipe_state(G.pre);
+    VB_HIP(hipStreamEndCapture(st, &gr));
+    const hipError_t e = hipGraphInstantiateWithFlags(&G.x, gr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      G.x = nullptr;
+      throw HipError{e, "hipGraphInstantiateWithFlags"};
+    }
+    return G;
+  }
+
+  StepGraph* find_graph() {
+    const PipeState now = pipe_state();
+    for (auto& G : graphs)
+      if (G.x && G.pre == now) return &G;
+    return nullptr;
+  }
+
+  // n fixed-dt iterations launched without host synchronisation.  A graph
+  // is captured once the pipeline is back in the state the previous fixed
+  // iteration left (the state that recurs from one iteration to the next)
+  // and replayed whenever the pipeline is in a captured graph's start state.
+  void launch_fixed(int n, vb_step_report& r) {
+    const bool use_graphs = p.graphs && !prof;
+    while (n > 0) {
+      if (use_graphs && n >= kGraphIters) {
+        StepGraph* G = find_graph();
+        if (!G && have_fixed_post && fixed_post == pipe_state()) G = &capture_iters();
+        if (G) {
+          VB_HIP(hipGraphLaunch(G->x, st));
+          set_pipe_state(G->post);
+          fixed_post = G->post;
+          r.evaluations += G->evals;
+          for (int i = 0; i < kGraphIters; ++i) {
+            time_current += time_step;
+            steps++;
+            r.steps++;
+          }
+          n -= kGraphIters;
+          continue;
+        }
+      }
+      r.evaluations += fixed_body();
+      fixed_post = pipe_state();
+      have_fixed_post = true;
+      time_current += time_step;
+      steps++;
+      r.steps++;
+      --n;
+    }
+  }
+
+  bool fixed_dt() const override { return p.fixed_time_step > 0 && p.scheme != VB_EULER; }
+
+  // ---- adaptive ETD iterations in two halves (belastic.cpp:639-669, 713-746)
+  // adaptive_issue() launches one attempt: the evaluation at the current dt,
+  // the energy of the candidate and the copy of its scalars to pinned host
+  // memory; adaptive_complete() waits for that copy and accepts or rejects.
+  // A scheduler can issue attempts of many solves before completing any.
+  hipEvent_t att_ev = nullptr;
+  bool att_pending = false;
+  bool iter_open = false;      // dt chosen for the current iteration
+  bool cur_E_known = false;    // cur_E is the energy of uh[cur] with the current targets
+  double cur_E = 0.0, att_eng0 = 0.0;
+  int att_keep = 0;
+
+  void adaptive_issue(vb_step_report& r) override {
+    if (!started) start();
+    if (!iter_open) {
+      if (!cur_E_known) {
+        ensure_prefixed();
+        read_scal();
+        cur_E = hs->E;
+        cur_E_known = true;
+      }
+      att_eng0 = cur_E;
+      if (steps % p.dt_double_every == 0) {
+        if (time_step * 1.4 < p.dt_cap) time_step = time_step * 2;
+        else time_step = p.dt_cap;
+      }
+      iter_open = true;
+    }
+    if (time_step < 1e-23) {
+      r.status = VB_ERR_DT_UNDERFLOW;
+      return;
+    }
+    if (!att_ev) VB_HIP(hipEventCreateWithFlags(&att_ev, hipEventDisableTiming));
+    const int out = eval(time_step);
+    att_keep = cur;
+    cur = out;
+    prefix(FIN_PLAIN, false);
+    VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipEventRecord(att_ev, st));
+    att_pending = true;
+  }
+
+  bool attempt_ready() override {
+    if (!att_pending) return true;
+    const hipError_t e = hipEventQuery(att_ev);
+    if (e == hipErrorNotReady) return false;
+    VB_HIP(e);
+    return true;
+  }
+
+  // 1: the iteration ended (accepted, or a driver exit in r.status); 0: the
+  // attempt was rejected and dt halved, issue again
+  int adaptive_complete(vb_step_report& r) override {
+    if (!att_pending) {
+      iter_open = false;
+      return 1;
+    }
+    VB_HIP(hipEventSynchronize(att_ev));
+    att_pending = false;
+    r.evaluations++;
+    if (hs->E > att_eng0) {
+      cur = att_keep;   // the state is untouched; only its prefix is redone
+      prefixed = prefixed_f = prepped = false;
+      time_step /= 2;
+      r.rejections++;
+      return 0;
+    }
+    iter_open = false;
+    cur_E = hs->E;
+    r.energy = hs->E;
+    if (att_eng0 < 1e-3) r.status = VB_FINISHED;          // belastic.cpp:727-732
+    else if (hs->nonfinite) r.status = VB_ERR_NONFINITE;  // belastic.cpp:733-740
+    if (r.status != VB_OK) return 1;
+    time_current += time_step;   // belastic.cpp:742-746
+    steps++;
+    r.steps++;
+    return 1;
+  }
+
+  void launch_steps(int n, vb_step_report* rep) override {
+    if (!started) start();
+    cur_E_known = false;
+    if (time_step < 1e-23) {
+      rep->status = VB_ERR_DT_UNDERFLOW;
+      return;
+    }
+    launch_fixed(n, *rep);
+  }
+
+  void finish_steps(vb_step_report* rep) override {
+    if (drift_mode()) sync_targets_from_device();
+    if (rep->status == VB_OK) {
+      ensure_prefixed();
+      read_scal();
+      if (hs->nonfinite) rep->status = VB_ERR_NONFINITE;
+    }
+    rep->time_step = time_step;
+    rep->time_current = time_current;
+  }
+
   void read_scal() {
     VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
@@ -537,6 +781,7 @@ struct Solver final : SolverBase {
 
   void start() {
     started = true;
+    cur_E_known = false;
     if (p.shape_adjust && time_current == 0.0) shape_adjust();
     ensure_prefixed(p.scheme == VB_EULER);
     if (beta <= 0.0) {   // belastic.cpp:588-591
@@ -564,6 +809,7 @@ struct Solver final : SolverBase {
     if (!started) start();
     const bool adaptive = p.fixed_time_step <= 0;
     const bool drift = !adaptive && (p.M1 != 0 || p.M2 != 0) && p.penalty_drift;
+    if (!adaptive || p.scheme == VB_EULER) cur_E_known = false;
     for (int it = 0; it < n; ++it) {
       double eng0 = NAN;
       if (p.scheme == VB_EULER) {
@@ -602,52 +848,19 @@ struct Solver final : SolverBase {
         if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
         if (time_current > p.tmax) { r.status = VB_FINISHED; break; }
       } else if (adaptive) {
-        // belastic.cpp:639-669
-        ensure_prefixed();
-        read_scal();
-        eng0 = hs->E;
-        if (steps % p.dt_double_every == 0) {
-          if (time_step * 1.4 < p.dt_cap) time_step = time_step * 2;
-          else time_step = p.dt_cap;
-        }
-        for (;;) {
-          if (time_step < 1e-23) break;
-          const int out = eval(time_step);
-          r.evaluations++;
-          const int keep = cur;
-          cur = out;
-          prefix(FIN_PLAIN, false);
-          read_scal();
-          if (hs->E > eng0) {
-            cur = keep;
-            prefixed = prefixed_f = prepped = false;
-            time_step /= 2;
-            r.rejections++;
-            ensure_prefixed();
-            continue;
-          }
-          r.energy = hs->E;
-          break;
-        }
-      } else if (drift) {
-        // belastic.cpp:670-691
-        ensure_prefixed();
-        int out = eval(time_step);
-        int keep = cur;
-        cur = out;
-        prefix(FIN_DRIFT, false);
-        cur = keep;
-        prepped = prefixed = prefixed_f = false;
-        ensure_prefixed();
-        out = eval(time_step);
-        cur = out;
-        prefix(FIN_DRIFT, false);
-        r.evaluations += 2;
+        // belastic.cpp:639-669: attempts until one is accepted
+        int acc = 0;
+        do {
+          adaptive_issue(r);
+          acc = adaptive_complete(r);
+        } while (!acc);
+        if (r.status != VB_OK) break;
+        continue;   // time, steps and the exits were handled on acceptance
       } else {
-        ensure_prefixed();
-        const int out = eval(time_step);
-        cur = out;
-        r.evaluations++;
+        // fixed dt (belastic.cpp:670-693): no host decision inside the loop
+        if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
+        launch_fixed(n - it, r);
+        break;
       }
       // exits (belastic.cpp:713-740)
       if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
@@ -735,6 +948,8 @@ struct Solver final : SolverBase {
   }
 
   void set_state(double a, double b, double dt, double t, double a0, double b0) override {
+    cur_E_known = false;
+    iter_open = false;
     alpha = a;
     beta = b;
     alpha0 = a0;
@@ -757,6 +972,8 @@ struct Solver final : SolverBase {
   }
 
   void set_phi(const double* host) override {
+    cur_E_known = false;
+    iter_open = false;
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
     L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
     L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, st); });
@@ -767,6 +984,7 @@ struct Solver final : SolverBase {
   }
 
   void evaluate(double h) override {
+    cur_E_known = false;
     const int out = eval(h);
     cur = out;
   }

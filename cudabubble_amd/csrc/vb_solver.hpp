@@ -25,6 +25,21 @@ struct SolverBase {
   virtual hipStream_t stream() = 0;
   virtual void profile(bool on) = 0;
   virtual int kernel_times(char* names, int name_len, int* counts, double* ms, int cap) = 0;
+  // Fixed-dt drivers only: launch_steps() issues n driver iterations without
+  // host synchronisation (graph replays once the pipeline recurs), and
+  // finish_steps() does what step() does after its loop (target read-back,
+  // blow-up test).  step(n) is launch_steps(n) + finish_steps() for them.
+  virtual bool fixed_dt() const = 0;
+  virtual void launch_steps(int n, vb_step_report* rep) = 0;
+  virtual void finish_steps(vb_step_report* rep) = 0;
+  // Adaptive ETD drivers: one attempt of the current iteration is issued
+  // without waiting (evaluation, candidate energy, scalars copied to the host);
+  // attempt_ready() polls it; adaptive_complete() accepts it (returns 1, time
+  // and steps advanced, exits in rep) or rejects it (returns 0, dt halved).
+  // step(n) on an adaptive solver is issue/complete until n acceptances.
+  virtual void adaptive_issue(vb_step_report& rep) = 0;
+  virtual bool attempt_ready() = 0;
+  virtual int adaptive_complete(vb_step_report& rep) = 0;
   // extra info stored in the checkpoint (.info): alpha3, alpha_1, beta_1
   double info_extra[3] = {0, 0, 0};
 };

@@ -65,7 +65,9 @@ typedef struct vb_params {
   int shape_adjust;        /* b_shape: Euler pre-relaxation at t = 0            */
   double tmax;             /* Euler driver stop time (belastic.cpp:407)         */
   int device;              /* HIP device ordinal                                */
-  int pad;
+  int graphs;              /* 1: fixed-dt iterations replay captured HIP graphs
+                              (same kernels and arguments as eager launches);
+                              0: eager launches only                            */
 } vb_params;
 
 /* Counters of one vb_step call. */
@@ -170,13 +172,67 @@ int vb_calc_sync(vb_calc* c);
 int vb_shape(int kind, unsigned long long seed, int nx, int nz, double len, double eps,
              double noise, double* phi_out, double* shape_params /* optional [8] */);
 
-/* ---- batched solves (many independent vesicles, one launch per pass) ------- */
+/* ---- batched solves ---------------------------------------------------------
+ * Many independent vesicles of any supported sizes on one GPU, each with its
+ * own parameters (no reference counterpart: the reference runs one belastic
+ * per process).  The solves are spread over a few streams; fixed-dt solves
+ * replay captured HIP graphs, two driver iterations per replay, issued
+ * round-robin.  Each solve computes exactly what vb_step computes for it. */
 typedef struct vb_batch vb_batch;
 int vb_batch_create(int n, const vb_params* params, const double* const* phi0, vb_batch** out);
+/* n driver iterations of every solve (vb_step semantics per solve) */
 int vb_batch_step(vb_batch* b, int n_steps);
+/* the vb_step_report of solve i from the last vb_batch_step */
+int vb_batch_report(vb_batch* b, int i, vb_step_report* rep);
 int vb_batch_query(vb_batch* b, int i, double* E, double* V, double* A, double* Eb);
 int vb_batch_get_phi(vb_batch* b, int i, double* host_out);
 void vb_batch_destroy(vb_batch* b);
+
+/* ---- request loop ----------------------------------------------------------
+ * Solves arrive at any time (no reference counterpart: the reference runs one
+ * belastic per process, main.cpp:17-18).  One worker thread owns the device:
+ * it admits queued requests in arrival order while their footprint fits the
+ * memory budget and a slot is free, keeps each admitted solve resident in HBM,
+ * and keeps one adaptive attempt of every running solve in flight on one of a
+ * few streams, deciding each on the host as soon as it completes.  A solve
+ * leaves when E dropped by at most converge_tol (relative) over the last
+ * converge_window accepted steps, when t >= tmax, at a driver exit
+ * (vb_step_report.status) or after max_steps.  Requests must use adaptive dt
+ * (fixed_time_step <= 0) and an ETD scheme.  Every result equals what vb_step
+ * computes for that request alone.  Client calls are thread-safe. */
+typedef struct vb_server_config {
+  int device;              /* HIP device ordinal                                  */
+  int max_active;          /* solves resident and stepping at once (16)          */
+  int streams;             /* device streams the running solves are spread over (4) */
+  int keep_phi;            /* 1: keep each final phi for vb_server_take_phi       */
+  size_t memory_budget;    /* bytes of HBM for resident solves; 0: 90% of free    */
+  int converge_window;     /* accepted steps of the energy-drop test (50)         */
+  int max_steps;           /* accepted-step cap per request, 0: none              */
+  double converge_tol;     /* relative energy drop (1e-6)                         */
+} vb_server_config;
+
+typedef struct vb_result {
+  long long id;
+  int status;              /* VB_OK when it left by convergence / tmax / cap      */
+  int steps, evaluations, rejections;
+  double E, V, A, Eb;      /* FP64 integrals of the final state                   */
+  double time_step, time_current;
+  double t_submit, t_admit, t_done;   /* seconds on the server clock           */
+} vb_result;
+
+typedef struct vb_server vb_server;
+void vb_server_config_default(vb_server_config* c);
+int vb_server_create(const vb_server_config* cfg, vb_server** out);
+/* queues a copy of phi0 ([nz][ny][nx] doubles) under the caller's id */
+int vb_server_submit(vb_server* s, long long id, const vb_params* p, const double* phi0);
+/* moves up to cap finished results to out; *n_out = how many */
+int vb_server_poll(vb_server* s, vb_result* out, int cap, int* n_out);
+int vb_server_pending(vb_server* s, int* queued, int* running, size_t* bytes_in_use);
+/* the final phi of a finished request (keep_phi = 1), once */
+int vb_server_take_phi(vb_server* s, long long id, double* host_out);
+double vb_server_clock(vb_server* s);
+/* finishes every queued and running request, then frees everything */
+void vb_server_destroy(vb_server* s);
 
 /* The ETD coefficient table the solver uses for step h (host only, no GPU):
  * out[m][8] for m = kx^2 + ky'^2 + kz'^2 in [0, mmax] with folded indices.

@@ -1,0 +1,86 @@
+"""GPU checks of graph-replayed fixed-dt steps and of batched solves: a graph
+replays exactly the launches the eager driver issues, so every result must be
+bit-identical to the eager, one-solve-at-a-time run of the same parameters."""
+import numpy as np
+import pytest
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def case(vb, nx, nz=None, kind=1, seed=7, **kw):
+    nz = nx if nz is None else nz
+    eps = kw.pop("eps", 2.0 * 2 * np.pi / nx)
+    phi, _ = vb.shape(kind, seed, nx, nz, 1.0, eps)
+    kw.setdefault("fixed_time_step", 5e-7)
+    kw.setdefault("time_step", 5e-7)
+    p = vb.params_default(nx=nx, nz=nz, eps=eps, **kw)
+    return p, phi
+
+
+def solo(vb, p, phi, chunks, graphs):
+    p.graphs = graphs
+    s = vb.Solver(p, phi)
+    reps = [s.step(c) for c in chunks]
+    return s.get_phi(), s.state(), s.query(), reps
+
+
+@pytest.mark.parametrize("prec,drift", [(0, 1), (1, 1), (0, 0), (1, 0)])
+def test_graph_replay_matches_eager(vb, gpu, prec, drift):
+    flush_print("graph vs eager", prec, drift)
+    M = 1e4 if drift else 0.0
+    p, phi = case(vb, 32, precision=prec, M1=M, M2=M)
+    chunks = [9, 1, 4, 7]
+    a = solo(vb, p, phi, chunks, graphs=1)
+    b = solo(vb, p, phi, chunks, graphs=0)
+    assert np.array_equal(a[0], b[0])
+    assert a[1] == b[1]
+    assert a[2] == b[2]
+    assert [r["evaluations"] for r in a[3]] == [r["evaluations"] for r in b[3]]
+    assert a[1]["time_current"] == b[1]["time_current"]
+
+
+def test_graph_replay_with_queries_between(vb, gpu):
+    # a query between calls changes the host-side pipeline state (plain mode)
+    p, phi = case(vb, 32, M1=0.0, M2=0.0)
+    outs = []
+    for g in (1, 0):
+        p.graphs = g
+        s = vb.Solver(p, phi)
+        q = []
+        for c in (4, 5, 6):
+            s.step(c)
+            q.append(s.query())
+        outs.append((s.get_phi(), q))
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
+
+
+def test_batch_matches_solo(vb, gpu):
+    specs = [
+        dict(nx=64, kind=3, seed=11, precision=1, M1=1e3, M2=3e4),
+        dict(nx=32, kind=1, seed=12, precision=1, M1=3e4, M2=1e3, eps=1.5 * 2 * np.pi / 32, phi_lut_mode=1),
+        dict(nx=48, nz=32, kind=2, seed=13, precision=0, M1=1e4, M2=1e4),
+        dict(nx=32, kind=4, seed=14, precision=1, M1=0.0, M2=0.0, eps=3.0 * 2 * np.pi / 32),
+        dict(nx=64, kind=1, seed=15, precision=0, M1=2e4, M2=5e3),
+        # adaptive dt: stepped with host decisions inside the batch
+        dict(nx=32, kind=1, seed=16, precision=0, M1=1e4, M2=1e4, fixed_time_step=0.0, time_step=1e-7),
+    ]
+    ps, phis = [], []
+    for sp in specs:
+        sp = dict(sp)
+        p, phi = case(vb, sp.pop("nx"), sp.pop("nz", None), sp.pop("kind"), sp.pop("seed"), **sp)
+        ps.append(p)
+        phis.append(phi)
+    flush_print("batch of", len(ps))
+    b = vb.Batch(ps, phis)
+    reps = b.step(5) + b.step(6)
+    got = [(b.get_phi(i), b.query(i)) for i in range(len(ps))]
+    b.close()
+    for i, (p, phi) in enumerate(zip(ps, phis)):
+        assert np.isfinite(got[i][0]).all() and reps[i]["status"] == 0, i
+        ref_phi, _, ref_q, ref_reps = solo(vb, p, phi, [5, 6], graphs=0)
+        assert np.array_equal(got[i][0], ref_phi), i
+        assert got[i][1] == ref_q, i
+        assert reps[i]["evaluations"] == ref_reps[0]["evaluations"], i
+        assert reps[len(ps) + i]["steps"] == ref_reps[1]["steps"], i
