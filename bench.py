@@ -180,6 +180,93 @@ def batched(vb, steps, seed=20260004):
             "groups": ["%d x %d^3" % (c, n) for n, c in BATCH_GROUPS]}
 
 
+SERVE_SIZES = ((64, 0.50), (128, 0.35), (256, 0.14), (512, 0.01))
+
+
+def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05):
+    """The serving form: Poisson arrivals at `rate` requests/s for `seconds`,
+    grid sizes drawn from {64^3: 50%, 128^3: 35%, 256^3: 14%, 512^3: 1%},
+    shapes and parameters drawn as in the batched form, FP32-mixed adaptive
+    ETDRK4 from dt = 1e-7 until the relative energy drop over 50 accepted steps
+    is below 1e-6 or t >= tmax = 1e-3.  Latency is submit -> result.  A 5%
+    sample is re-run in FP64 (same request, one at a time) for the error."""
+    rng = np.random.default_rng(seed)
+    t_arr, reqs = 0.0, []
+    while True:
+        t_arr += rng.exponential(1.0 / rate)
+        if t_arr > seconds:
+            break
+        n = int(rng.choice([s for s, _ in SERVE_SIZES], p=[q for _, q in SERVE_SIZES]))
+        j = len(reqs)
+        kind = (vb.SHAPE_ELLIPSOID, vb.SHAPE_TORUS, vb.SHAPE_DISK)[j % 3]
+        eps = (1.5, 2.0, 3.0)[int(rng.integers(3))] * (2 * np.pi) / n
+        M1, M2 = 10.0 ** rng.uniform(3, 5, size=2)
+        reqs.append((t_arr, n, kind, eps, M1, M2))
+
+    def make(j, precision):
+        _, n, kind, eps, M1, M2 = reqs[j]
+        phi, _ = vb.shape(kind, seed + j, n, n, 1.0, eps)
+        p = vb.params_default(nx=n, nz=n, eps=eps, M1=M1, M2=M2, alpha=0.0, beta=0.0, time_step=1e-7,
+                              fixed_time_step=0.0, precision=precision, phi_lut_mode=vb.LUT_ACCURATE,
+                              scheme=vb.ETDRK4, tmax=1e-3)
+        return p, phi
+
+    srv = vb.Server(max_active=16, streams=4, converge_window=50, converge_tol=1e-6)
+    results, t0 = {}, time.perf_counter()
+    for j, (ta, *_rest) in enumerate(reqs):
+        p, phi = make(j, vb.FP32_MIXED)      # prepared before its arrival time
+        while time.perf_counter() - t0 < ta:
+            for r in srv.poll():
+                results[r["id"]] = r
+            time.sleep(0.0005)
+        srv.submit(j, p, phi)
+    for r in srv.drain():
+        results[r["id"]] = r
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in results.values()])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in results.values()])
+    gp_evals = sum(reqs[i][1] ** 3 * r["evaluations"] for i, r in results.items())
+    span = max(r["t_done"] for r in results.values()) - min(r["t_submit"] for r in results.values())
+    # FP64 re-runs of a sample, each alone
+    sample = sorted(rng.choice(len(reqs), size=max(1, int(round(fp64_sample * len(reqs)))), replace=False))
+    errs = []
+    for j in sample:
+        p, phi = make(int(j), vb.FP64)
+        s = vb.Solver(p, phi)
+        steps = 0
+        hist = []
+        while True:
+            r = s.step(1)
+            steps += 1
+            hist.append(r["energy"])
+            if r["status"] != 0 or r["time_current"] >= p.tmax:
+                break
+            if len(hist) > 50 and abs(hist[-51] - hist[-1]) <= 1e-6 * abs(hist[-51]):
+                break
+        E, V, A, Eb = s.query()
+        s.close()
+        q = results[int(j)]
+        errs.append({"id": int(j), "n": reqs[int(j)][1], "E_rel": abs(q["E"] - E) / abs(E),
+                     "V_rel": abs(q["V"] - V) / abs(V), "A_rel": abs(q["A"] - A) / abs(A),
+                     "steps_fp32": q["steps"], "steps_fp64": steps})
+    sizes = {}
+    for i in results:
+        sizes[reqs[i][1]] = sizes.get(reqs[i][1], 0) + 1
+    status = {}
+    for r in results.values():
+        status[str(r["status"])] = status.get(str(r["status"]), 0) + 1
+    return {"requests": len(reqs), "completed": sum(1 for r in results.values() if r["status"] == 0),
+            "status_counts": status,
+            "arrival_seconds": seconds, "rate_per_s": rate, "sizes": {str(k): v for k, v in sorted(sizes.items())},
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+            "queue_wait_s": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
+            "aggregate_gp_steps_per_s": gp_evals / span, "wall_s": wall,
+            "mean_steps": float(np.mean([r["steps"] for r in results.values()])),
+            "fp64_sample": errs}
+
+
 def copy_calibration():
     """A plain device copy of known size (256 MiB read + 256 MiB written), the
     reference point for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -204,6 +291,8 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--batch-steps", type=int, default=300,
                     help="steps of the 48-solve batched form (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="arrival window of the serving form at 20 requests/s (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -281,6 +370,8 @@ def main():
         if "fp64" in out_extra:
             bt["ratio_vs_fp64_single_%d" % n] = bt["value"] / out_extra["fp64"]["value"]
         out_extra["batched"] = bt
+    if args.serve_seconds > 0:
+        out_extra["serving"] = serving(vb, args.serve_seconds)
     cpu = None if args.no_cpu else cpu_baseline(n)
 
     line = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,

@@ -109,6 +109,9 @@ def lib():
     L.vb_get_phi.argtypes = [P, P]
     L.vb_set_phi.argtypes = [P, P]
     L.vb_evaluate.argtypes = [P, D]
+    L.vb_residual.argtypes = [P, C.POINTER(D)]
+    L.vb_shape_ninit.argtypes = [I, I, I, I, D, D, P]
+    L.vb_read_field.argtypes = [C.c_char_p, I, I, I, P]
     L.vb_save.argtypes = [P, C.c_char_p]
     L.vb_load.argtypes = [P, C.c_char_p]
     L.vb_stream.restype = P
@@ -217,6 +220,27 @@ def shape(kind, seed, nx, nz=None, length=1.0, eps=None, noise=0.0):
     return phi, sp
 
 
+def shape_ninit(ninit, nx, nz=None, length=1.0, eps=None):
+    """The reference's deterministic initial shape of ini field ninit (0, 1, 2, 4)."""
+    nz = nx if nz is None else nz
+    if eps is None:
+        eps = 2.0 * (2 * np.pi * length) / nx
+    phi = np.zeros((nz, nx, nx))
+    _check(lib().vb_shape_ninit(int(ninit), nx, nx, nz, float(length), float(eps), _np_ptr(phi)), "vb_shape_ninit")
+    return phi
+
+
+def read_field(path, nx, nz=None):
+    """A raw [nz][ny][nx] field file; (nx+1) x (ny+1) x nz files are trimmed."""
+    nz = nx if nz is None else nz
+    phi = np.zeros((nz, nx, nx))
+    _check(lib().vb_read_field(os.fsencode(path), nx, nx, nz, _np_ptr(phi)), "vb_read_field")
+    return phi
+
+
+HOST_PROGRAM = os.path.join(HERE, "cudabubble")
+
+
 class Solver:
     """One vesicle solve (belastic + phasefield, belastic.cpp:15-23)."""
 
@@ -247,6 +271,12 @@ class Solver:
 
     def evaluate(self, h):
         _check(lib().vb_evaluate(self.h, float(h)), "vb_evaluate")
+
+    def residual(self):
+        """LSerror of the Euler right-hand side (belastic::geterror)."""
+        e = C.c_double()
+        _check(lib().vb_residual(self.h, C.byref(e)), "vb_residual")
+        return e.value
 
     def query(self):
         E, V, A, Eb = C.c_double(), C.c_double(), C.c_double(), C.c_double()

@@ -79,7 +79,7 @@ def build_host_programs():
         return
     exe = os.path.join(HERE, "cudabubble")
     cmd = ["g++", "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + HOST, "-o", exe] + srcs + \
-          ["-L" + HERE, "-lvesicle", "-Wl,-rpath,$ORIGIN"]
+          ["-L" + HERE, "-lvesicle", "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
     rc, c, out = _run(cmd)
     if rc != 0:
         raise RuntimeError("host program build failed: %s\n%s" % (c, out))

@@ -281,6 +281,14 @@ int vb_evaluate(vb_solver* s, double h) {
   });
 }
 
+int vb_residual(vb_solver* s, double* err) {
+  if (!s || !s->impl || !err) return VB_ERR_PARAM;
+  return guard([&] {
+    *err = s->impl->residual();
+    return (int)VB_OK;
+  });
+}
+
 void* vb_stream(vb_solver* s) { return (s && s->impl) ? (void*)s->impl->stream() : nullptr; }
 
 int vb_profile(vb_solver* s, int on) {

@@ -281,6 +281,7 @@ struct XeArgs {
   double eps, eta, t, inv_n;
   int shape;         // 1: shape_adjust (rhs = g0), 0: run_static (rhs = nld1)
   Rows rl;
+  double* part;      // non-null: only sum rhs^2 per block here (no update)
 };
 
 template <typename T, int N>
@@ -298,6 +299,7 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, j + TF * e);
   fft<T, N, +1>(x, j, a.tw, lds);
   const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.t;
+  double acc[1] = {0.0};
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = j + TF * e;
@@ -310,8 +312,14 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
       const double g0 = -lf + (3.0 * v * v - 1.0) * f / (eps * eps);    // Calc.cu:178
       const double rhs = a.shape ? g0 : l1 * 0.5 + l2 * f + g0 * eta;    // Calc.cu:417
       nv[h] = v + t * rhs;                                               // Calc.cu:400
+      acc[0] += valid ? rhs * rhs : 0.0;
     }
     x[e] = mk<T>(T(nv[0]), T(nv[1]));
+  }
+  if (a.part) {   // uniform over the grid: the residual sum only
+    __shared__ double red[XWG / 64];
+    block_sum_store<1, XWG>(acc, red, a.part + blockIdx.x);
+    return;
   }
   fft<T, N, -1>(x, j, a.tw, lds);
   unpack_store<T, N>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);

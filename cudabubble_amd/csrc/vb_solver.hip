@@ -139,6 +139,7 @@ struct Solver final : SolverBase {
   DBuf<double> v64, f64, xpart64, zpart64, dc64;
   DBuf<Scal> scal64;
   Scal* hs = nullptr;   // pinned host copy
+  DBuf<double> rpart;   // block sums of the Euler residual
   int nxb = 0, nzb = 0;
   long long npairs = 0;
   int mmax = 0;
@@ -504,6 +505,7 @@ struct Solver final : SolverBase {
     e.inv_n = 1.0 / double(g.nr);
     e.shape = shape ? 1 : 0;
     e.rl = Rows{g.ny, g.pitch, g.pp};
+    e.part = nullptr;
     L("xe", [&] { ox->xe(e, st); });
     L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
@@ -981,6 +983,38 @@ struct Solver final : SolverBase {
     check_launch();
     VB_HIP(hipStreamSynchronize(st));
     prepped = prefixed = prefixed_f = false;
+  }
+
+  // belastic::geterror (belastic.cpp:136-143): I[r^2; 0, 1] of the Euler
+  // right-hand side r = lambda1/2 + lambda2 f + eta g0 (belastic.cpp:52-64)
+  double residual() override {
+    if (!started) start();
+    if (!W.p) W.alloc((size_t)g.ncp);
+    if (!rpart.p) rpart.alloc((size_t)nxb);
+    ensure_prefixed(true);
+    XeArgs<T> e;
+    e.W = W.p;
+    e.v = v.p;
+    e.f = f.p;
+    e.out = nullptr;
+    e.sc = scal.p;
+    e.tw = twx.p;
+    e.npairs = npairs;
+    e.eps = p.eps;
+    e.eta = p.eta;
+    e.t = 0.0;
+    e.inv_n = 1.0 / double(g.nr);
+    e.shape = 0;
+    e.rl = Rows{g.ny, g.pitch, g.pp};
+    e.part = rpart.p;
+    L("xe_resid", [&] { ox->xe(e, st); });
+    check_launch();
+    std::vector<double> h((size_t)nxb);
+    VB_HIP(hipMemcpyAsync(h.data(), rpart.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+    double sum = 0.0;
+    for (double x : h) sum += x;
+    return sum / double(g.nr) * g.L3;   // Calc.cu:250,260 with a = 0, m = 1
   }
 
   void evaluate(double h) override {

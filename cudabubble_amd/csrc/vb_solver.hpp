@@ -22,6 +22,7 @@ struct SolverBase {
   virtual void get_phi(double* host) = 0;
   virtual void set_phi(const double* host) = 0;
   virtual void evaluate(double h) = 0;
+  virtual double residual() = 0;
   virtual hipStream_t stream() = 0;
   virtual void profile(bool on) = 0;
   virtual int kernel_times(char* names, int name_len, int* counts, double* ms, int cap) = 0;

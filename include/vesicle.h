@@ -110,6 +110,10 @@ int vb_query(vb_solver* s, double* E, double* V, double* A, double* Eb);
 int vb_state(vb_solver* s, double* alpha, double* beta, double* time_step, double* time_current);
 int vb_get_phi(vb_solver* s, double* host_out);        /* cudaMemcpy D2H, belastic.cpp:163 */
 int vb_set_phi(vb_solver* s, const double* host_in);   /* phasefield ctor H2D copy        */
+/* belastic::geterror (belastic.cpp:136-143): LSerror of the Euler right-hand
+ * side lambda1/2 + lambda2 f + eta g0 of the current field (belastic.cpp:52-64),
+ * the "error" column of the reference's output */
+int vb_residual(vb_solver* s, double* err);
 /* one stepper evaluation without the driver: ETD4RK / ETD2RK
  * (belastic.cpp:519-552, 491-517) with step h */
 int vb_evaluate(vb_solver* s, double h);
@@ -171,6 +175,15 @@ int vb_calc_sync(vb_calc* c);
  * in the ranges of the benchmark configurations; noise > 0 adds N(0, noise). */
 int vb_shape(int kind, unsigned long long seed, int nx, int nz, double len, double eps,
              double noise, double* phi_out, double* shape_params /* optional [8] */);
+
+/* The reference's deterministic initial shapes by ini field ninit
+ * (fluidbase.cpp:539-552): 0 sphere (:1080-1099), 1 two spheres (:1197-1215),
+ * 2 spheroid (:1862-1877), 4 torus of varying tube radius (:2068-2090). */
+int vb_shape_ninit(int ninit, int nx, int ny, int nz, double len, double eps, double* phi_out);
+/* A raw field file as write_array stores it (tool.cpp:28-42): nx*ny*nz
+ * doubles, or (nx+1)*(ny+1)*nz doubles whose last x and y planes repeat the
+ * first (the 65x65x64 fields of the reference's results folder), trimmed. */
+int vb_read_field(const char* path, int nx, int ny, int nz, double* out);
 
 /* ---- batched solves ---------------------------------------------------------
  * Many independent vesicles of any supported sizes on one GPU, each with its
