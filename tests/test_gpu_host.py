@@ -53,9 +53,9 @@ def test_host_program_run_and_resume(vb, gpu, tmp_path):
     info = np.fromfile(fold / "32x32x32.info", dtype="<f8")
     st = s.state()
     assert info[0] == st["time_current"] and info[2] == st["alpha"] and info[3] == st["beta"]
-    # the energy log: first line at t = 0, last at step 20
+    # the energy log: first line at t = 0, last at t = 20 dt, stamped in units of 1e-6
     rows = [l.split(",") for l in (fold / "energy32x32x32.txt").read_text().splitlines()]
-    assert int(rows[0][0]) == 0 and int(rows[-1][0]) == 20
+    assert int(rows[0][0]) == 0 and int(rows[-1][0]) == round(20 * 1e-7 / 1e-6)
     E, V, A, Eb = s.query()
     assert abs(float(rows[-1][7]) - E) < 1e-5 * max(1.0, abs(E))
     # resume: the .info makes the next run continue from t = 20 dt; the same
@@ -66,6 +66,6 @@ def test_host_program_run_and_resume(vb, gpu, tmp_path):
     r = run_cli(vb, tmp_path, "--max-steps", "10")
     assert r.returncode == 0 and "Continue the existing experiment" in r.stdout
     info2 = np.fromfile(fold / "32x32x32.info", dtype="<f8")
-    assert abs(info2[0] - 30 * 1e-6) < 1e-12 and info2[0] == s2.state()["time_current"]
+    assert abs(info2[0] - 30 * 1e-7) < 1e-13 and info2[0] == s2.state()["time_current"]
     cli_phi = np.fromfile(fold / "d32x32x32.out", dtype="<f8").reshape(32, 32, 32)
     assert np.array_equal(cli_phi, s2.get_phi())

@@ -60,7 +60,7 @@ def test_read_field_trims_repeated_planes(vb, tmp_path):
         vb.read_field(str(h), n, nz)
 
 
-INI = ("101  32  32  32   1  1  1   0   1e-6  5e-6  1e-6   2  -1  1  200  0  30  1.0  1  1  -1  -1  "
+INI = ("101  32  32  32   1  1  1   0   1e-7  5e-6  1e-7   2  -1  1  200  0  30  1.0  1  1  -1  -1  "
        "1.0e4  1.0e4  0  0  100.0  30  0.05  0  30.0  70  10  -1\n")
 
 
