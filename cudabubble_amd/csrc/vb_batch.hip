@@ -2,15 +2,16 @@
 //
 // Every solve keeps its state, work buffers and coefficient tables resident in
 // HBM for the life of the batch.  Solves are spread over a few streams by
-// longest-processing-time assignment (grid points per stream balanced), and a
-// batch step issues the solves' fixed-dt iterations round-robin, two at a time,
-// each pair one replay of the solve's captured graph.  Small grids, whose
-// kernels cannot fill 256 CUs, then run concurrently with the large ones, and
-// the host issues one graph launch per solve and pair of iterations instead
-// of about sixty kernel launches per iteration.  Solves with adaptive dt need
+// longest-processing-time assignment (grid points per stream balanced).  A
+// batch step runs each stream's solves one after the other, all steps of one
+// solve before the next (its working set stays cached between steps), the
+// streams concurrently; fixed-dt iterations replay the solve's captured graph,
+// two iterations per replay, so the host issues one graph launch where the
+// eager driver would issue about sixty kernel launches per iteration.  Solves with adaptive dt need
 // a host decision every iteration; they step after the fixed-dt work has been
 // issued and overlap with it.
 #include <algorithm>
+#include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -19,13 +20,20 @@
 #include "vb_solver.hpp"
 
 namespace {
-constexpr int kBatchStreams = 4;   // one per hardware queue of the process
+// One stream per hardware queue the runtime gives the process
+// (GPU_MAX_HW_QUEUES, 4 unless set): streams beyond that share queues.
+int batch_streams() {
+  const char* e = getenv("GPU_MAX_HW_QUEUES");
+  const int q = e ? atoi(e) : 0;
+  return q >= 1 ? std::min(q, 32) : 4;
+}
 }
 
 struct vb_batch {
   std::vector<std::unique_ptr<vb::SolverBase>> solves;
   std::vector<int> order;                 // decreasing grid size
   std::vector<hipStream_t> streams;
+  std::vector<int> stream_of;             // stream index of each solve
   std::vector<vb_step_report> reports;    // of the last vb_batch_step
   ~vb_batch() {
     solves.clear();
@@ -69,7 +77,7 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
   return bguard([&] {
     auto b = std::make_unique<vb_batch>();
     VB_HIP(hipSetDevice(params[0].device));
-    const int ns = std::min(n, kBatchStreams);
+    const int ns = std::min(n, batch_streams());
     for (int k = 0; k < ns; ++k) {
       hipStream_t s;
       VB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -79,15 +87,15 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int a, int c) { return points(params[a]) > points(params[c]); });
-    std::vector<int> stream_of(n);
+    b->stream_of.assign(n, 0);
     std::vector<long long> load(ns, 0);
     for (int i : b->order) {
       const int k = int(std::min_element(load.begin(), load.end()) - load.begin());
-      stream_of[i] = k;
+      b->stream_of[i] = k;
       load[k] += points(params[i]);
     }
     for (int i = 0; i < n; ++i) {
-      b->solves.push_back(vb::make_solver(params[i], b->streams[stream_of[i]]));
+      b->solves.push_back(vb::make_solver(params[i], b->streams[b->stream_of[i]]));
       b->solves.back()->set_phi(phi0[i]);
     }
     b->reports.assign(n, vb_step_report{});
@@ -103,13 +111,31 @@ int vb_batch_step(vb_batch* b, int n_steps) {
       r = vb_step_report{};
       r.energy = NAN;
     }
-    // fixed-dt solves: iterations issued round-robin, a graph replay at a time
-    constexpr int chunk = 2;
-    for (int done = 0; done < n_steps; done += chunk) {
-      const int k = std::min(chunk, n_steps - done);
-      for (int i : b->order) {
-        auto& s = b->solves[(size_t)i];
-        if (s->fixed_dt() && b->reports[(size_t)i].status == VB_OK) s->launch_steps(k, &b->reports[(size_t)i]);
+    // Fixed-dt solves, solve-major on each stream: a stream runs all n steps
+    // of one solve before the next, so that solve's working set stays in the
+    // caches from one step to the next.  The host issues a chunk of steps per
+    // stream in turn, which keeps every stream fed without filling one
+    // stream's queue while the others wait.
+    constexpr int chunk = 16;
+    const size_t ns = b->streams.size();
+    std::vector<std::vector<int>> lists(ns);
+    for (int i : b->order)
+      if (b->solves[(size_t)i]->fixed_dt()) lists[(size_t)b->stream_of[(size_t)i]].push_back(i);
+    std::vector<size_t> cur(ns, 0);
+    std::vector<int> done(ns, 0);
+    for (bool any = true; any;) {
+      any = false;
+      for (size_t k = 0; k < ns; ++k) {
+        if (cur[k] >= lists[k].size()) continue;
+        any = true;
+        const int i = lists[k][cur[k]];
+        const int c = std::min(chunk, n_steps - done[k]);
+        if (c > 0 && b->reports[(size_t)i].status == VB_OK) b->solves[(size_t)i]->launch_steps(c, &b->reports[(size_t)i]);
+        done[k] += std::max(c, 0);
+        if (done[k] >= n_steps || b->reports[(size_t)i].status != VB_OK) {
+          ++cur[k];
+          done[k] = 0;
+        }
       }
     }
     // adaptive solves decide on the host every iteration
