@@ -24,6 +24,11 @@ template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
 
 // Column tile width of the y / z kernels (columns that are contiguous in
 // memory): enough columns for >= 128-byte segments and >= 256 threads.
+// Minimum waves per SIMD requested for the fused x kernels: in FP32 four
+// (the register budget that allows it costs at most a few spilled words and
+// gains more in latency hiding than it loses); FP64 keeps the full budget.
+template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
+
 template <typename T, int N> constexpr int tbw() {
   constexpr int seg = 128 / int(sizeof(cpx<T>));
   constexpr int occ = 256 / TFof<N>();
@@ -145,7 +150,7 @@ struct XaArgs {
 };
 
 template <typename T, int N, bool WITHF>
-__global__ void __launch_bounds__(XWG) k_xa(XaArgs<T> a) {
+__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
   __shared__ double red[3 * (XWG / 64)];
@@ -222,7 +227,7 @@ struct XbArgs {
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(XWG) k_xb(XbArgs<T> a) {
+__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
@@ -587,6 +592,18 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a) {
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
+  constexpr bool USES_U = !(SCH == 4 && STG == 3);
+  cpx<double> ur[USES_U ? E : 1];
+  cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? E : 1];
+  cpx<T> pb[(SCH == 4 && STG == 4) ? E : 1];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const long long i = id.q + (long long)(j + TF * e) * plane;
+    if constexpr (USES_U) ur[e] = a.uh[i];
+    if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[i];
+    if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[i]; pb[e] = a.Nb[i]; }
+    if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[i];
+  }
   fft<T, N, -1>(x, j, a.tw, lds);   // x = N^ of this stage
   cpx<double> nf[E];
 #pragma unroll
@@ -598,41 +615,41 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a) {
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
       const double2 c = a.c1[m];                      // e1, q
-      const cpx<double> u = a.uh[i];
+      const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd41: a^
       fi = c.x * u.y + c.y * ni;
       if (id.valid) a.pre[i] = mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni));
     } else if constexpr (SCH == 4 && STG == 2) {
       const double2 c = a.c1[m];
-      const cpx<double> u = a.uh[i];
+      const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd42: b^
       fi = c.x * u.y + c.y * ni;
       if (id.valid) a.Na[i] = x[e];
     } else if constexpr (SCH == 4 && STG == 3) {
       const double q = a.c1[m].y;
-      const cpx<T> p = a.pre[i];
+      const cpx<T> p = pa[e];
       fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
       fi = double(p.y) + 2.0 * q * ni;
       if (id.valid) a.Nb[i] = x[e];
     } else if constexpr (SCH == 4 && STG == 4) {
       const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
-      const cpx<double> u = a.uh[i];
-      const cpx<T> na = a.Na[i], nb = a.Nb[i];
+      const cpx<double> u = ur[e];
+      const cpx<T> na = pa[e], nb = pb[e];
       // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
       fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
       fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
     } else if constexpr (SCH == 2 && STG == 1) {
       const double2 c = a.c1[m];                      // e^z, q1
-      const cpx<double> u = a.uh[i];
+      const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd1: a^
       fi = c.x * u.y + c.y * ni;
       if (id.valid) a.Na[i] = x[e];
     } else {                                          // SCH == 2 && STG == 2
       const double2 c = a.c1[m];
       const double c2 = a.c2[m].x;
-      const cpx<double> u = a.uh[i];
-      const cpx<T> nu = a.Na[i];
+      const cpx<double> u = ur[e];
+      const cpx<T> nu = pa[e];
       // etd2: u^+ = F(a) + c2 (N^a - N^u), a^ = e^z u^ + q1 N^u
       const double ar = c.x * u.x + c.y * double(nu.x), ai = c.x * u.y + c.y * double(nu.y);
       fr = ar + c2 * (nr - double(nu.x));

@@ -35,17 +35,19 @@ struct FinArgs {
   Phys ph;
 };
 
-__global__ void __launch_bounds__(256) k_finalize(FinArgs a) {
-  __shared__ double red[4 * 4];
+constexpr int kFinThreads = 1024;   // one block; wide so the partial sums load in few rounds
+
+__global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a) {
+  __shared__ double red[4 * (kFinThreads / 64)];
   __shared__ double tot[4];
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < a.nxb; i += 256) {
+  for (int i = threadIdx.x; i < a.nxb; i += kFinThreads) {
     s[0] += a.xpart[3 * i];
     s[1] += a.xpart[3 * i + 1];
     s[2] += a.xpart[3 * i + 2];
   }
-  for (int i = threadIdx.x; i < a.nzb; i += 256) s[3] += a.zpart[i];
-  block_sum_store<4, 256>(s, red, tot);
+  for (int i = threadIdx.x; i < a.nzb; i += kFinThreads) s[3] += a.zpart[i];
+  block_sum_store<4, kFinThreads>(s, red, tot);
   __syncthreads();
   if (threadIdx.x == 0) {
     Scal& c = *a.sc;
@@ -105,7 +107,7 @@ void launch_finalize(const double* xpart, int nxb, const double* zpart, const do
   a.pad = 0;
   a.g = g;
   a.ph = ph;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, a);
 }
 
 void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s) {
