@@ -186,8 +186,10 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a) {
         const double t1 = v * (v * v - 1.0 - kappa);       // nld_kernel1, Calc.cu:270
         if (valid) {
           acc[0] += v; acc[1] += w * w; acc[2] += f * f;
-          a.v[r * N + p] = T(v);
-          a.f[r * N + p] = T(f);
+          if constexpr (WITHF) {   // the Euler update reads v and f back; xb recomputes them
+            a.v[r * N + p] = T(v);
+            a.f[r * N + p] = T(f);
+          }
         }
         tm[h][e] = T(t1);
         if constexpr (WITHF) ff[h][e] = T(f);
@@ -217,8 +219,8 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a) {
 template <typename T>
 struct XbArgs {
   cpx<T>* Tt;        // in: x-half-spectra of lap tmp1; out: x-forward of N, times inv_n
-  const T* v;
-  const T* f;
+  const cpx<T>* P;   // x-half-spectra of v and lap v, as xa read them: v and f are
+  const cpx<T>* Q;   // recomputed here instead of being stored by xa and read back
   const Scal* sc;
   const cpx<T>* tw;
   long long npairs;
@@ -229,38 +231,37 @@ struct XbArgs {
 template <typename T, int N>
 __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a) {
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
-  __shared__ cpx<T> sm[G * LL];
+  __shared__ cpx<T> sm[2 * G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
+  RowLds<T> lds2{sm + (G + lane) * LL};
   long long pair = (long long)blockIdx.x * G + lane;
   const bool valid = pair < a.npairs;
   if (!valid) pair = a.npairs - 1;
   const long long r0 = 2 * pair;
-  cpx<T> x[E];
-  T vv[2][E], ffv[2][E];   // prefetched before the transform so the loads overlap it
+  cpx<T> x[E], vq[2][E];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      vv[h][e] = a.v[(r0 + h) * N + j + TF * e];
-      ffv[h][e] = a.f[(r0 + h) * N + j + TF * e];
-    }
+    for (int e = 0; e < E; ++e)
+      vq[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.Q + a.rl.off(r0 + h), j + TF * e);
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, j + TF * e);
-  fft<T, N, +1>(x, j, a.tw, lds);
+  fft2<T, N, +1>(vq[0], vq[1], j, a.tw, lds, lds2);   // v + i lap v of rows r0, r0 + 1
+  fft<T, N, +1>(x, j, a.tw, lds);                     // lap tmp1 of rows r0 (re), r0 + 1 (im)
   const double l1 = a.sc->l1, l2 = a.sc->l2;
-  const double ie = 1.0 / a.eps, ie2 = ie * ie, ie3 = ie * ie2;
+  const double eps = a.eps, ie = 1.0 / eps, ie2 = ie * ie, ie3 = ie * ie2;
   const double D = a.D, kappa = a.kappa, eta = a.eta;
   const double c32 = 2.0 * sqrt(2.0);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int p = j + TF * e;
     double nn[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const double dt1 = double(h ? x[e].y : x[e].x);
-      const double v = double(vv[h][e]), f = double(ffv[h][e]);
+      const double v = double(vq[h][e].x), lv = double(vq[h][e].y);
+      const double f = -lv * eps + (v * v - 1.0) * v / eps;                          // Calc.cu:160
       const double t1 = v * (v * v - 1.0 - kappa);
       const double nl = -dt1 * ie + D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f;   // Calc.cu:282
       nn[h] = -l1 * 0.5 - l2 * f * 3.0 / c32 - nl * eta;                           // Calc.cu:435

@@ -432,8 +432,8 @@ struct Solver final : SolverBase {
   void suffix(int sch, int stg, const Lut* lt, int out) {
     XbArgs<T> b;
     b.Tt = Tt.p;
-    b.v = v.p;
-    b.f = f.p;
+    b.P = P.p;
+    b.Q = Q.p;
     b.sc = scal.p;
     b.tw = twx.p;
     b.npairs = npairs;
