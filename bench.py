@@ -365,6 +365,10 @@ def main():
     out_extra["kernels_fp32"] = [{"name": nm, "launches": c, "ms": round(m, 3),
                                   "GBps": (bb / (m / c * 1e-3) / 1e9) if bb else None}
                                  for m, nm, c, bb in prof32]
+    # the single-solve measurements are done: free their HBM before the other forms
+    s32.close()
+    if not args.no_fp64:
+        s64.close()
     if args.batch_steps > 0:
         bt = batched(vb, args.batch_steps)
         if "fp64" in out_extra:

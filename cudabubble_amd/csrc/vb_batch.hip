@@ -11,7 +11,6 @@
 // a host decision every iteration; they step after the fixed-dt work has been
 // issued and overlap with it.
 #include <algorithm>
-#include <cstdlib>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -20,13 +19,9 @@
 #include "vb_solver.hpp"
 
 namespace {
-// One stream per hardware queue the runtime gives the process
-// (GPU_MAX_HW_QUEUES, 4 unless set): streams beyond that share queues.
-int batch_streams() {
-  const char* e = getenv("GPU_MAX_HW_QUEUES");
-  const int q = e ? atoi(e) : 0;
-  return q >= 1 ? std::min(q, 32) : 4;
-}
+// Four streams, each with a hardware queue of its own: measured best for
+// mixed sizes (more concurrent solves thrash the caches).
+constexpr int kBatchStreams = 4;
 }
 
 struct vb_batch {
@@ -77,11 +72,9 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
   return bguard([&] {
     auto b = std::make_unique<vb_batch>();
     VB_HIP(hipSetDevice(params[0].device));
-    const int ns = std::min(n, batch_streams());
+    const int ns = std::min(n, kBatchStreams);
     for (int k = 0; k < ns; ++k) {
-      hipStream_t s;
-      VB_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-      b->streams.push_back(s);
+      b->streams.push_back(vb::make_queue_stream());
     }
     b->order.resize(n);
     std::iota(b->order.begin(), b->order.end(), 0);

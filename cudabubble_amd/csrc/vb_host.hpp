@@ -64,6 +64,13 @@ struct DBuf {
 
 Geo make_geo(int nx, int nz, double L);
 
+// A stream with a hardware queue of its own: created with a CU mask that
+// enables every CU of the current device.  Plain streams are mapped onto the
+// process's few hardware queues in launch order, so streams created after
+// others have launched work can end up sharing a queue and serialise; the
+// concurrent streams of a batch or a request loop must not.
+hipStream_t make_queue_stream();
+
 // finalize kernel launcher (vb_solver.hip)
 void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
                      const Geo& g, const Phys& ph, int mode, hipStream_t s);

@@ -246,9 +246,7 @@ int vb_server_create(const vb_server_config* cfg, vb_server** out) {
       s->cfg.memory_budget = fr / 10 * 9;
     }
     for (int k = 0; k < cfg->streams; ++k) {
-      hipStream_t st;
-      VB_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-      s->streams.push_back(st);
+      s->streams.push_back(vb::make_queue_stream());
       s->stream_load.push_back(0);
     }
     vb_server* raw = s.get();

@@ -110,6 +110,18 @@ void launch_finalize(const double* xpart, int nxb, const double* zpart, const do
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, a);
 }
 
+hipStream_t make_queue_stream() {
+  int dev = 0;
+  VB_HIP(hipGetDevice(&dev));
+  int ncu = 0;
+  VB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+  for (int c = 0; c < ncu; ++c) mask[(size_t)c / 32] |= 1u << (c % 32);
+  hipStream_t s = nullptr;
+  VB_HIP(hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()));
+  return s;
+}
+
 void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_scal, dim3(1), dim3(64), 0, s, sc, alpha, beta, alpha0, beta0);
 }
