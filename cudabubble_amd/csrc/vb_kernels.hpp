@@ -579,8 +579,15 @@ struct ZsArgs {
 // The N^ of a stage leaves the FFT in precision T and is stored as such, so
 // keeping it for a later stage loses nothing; the state and every combine
 // that produces it are FP64.
+// Minimum waves per SIMD for the stage combines: the FP32 ETDRK4 stages 1
+// and 3 fit four waves with a few spilled words and run faster so; stages 2
+// and 4 (FP64 state and two stage spectra live) would spill heavily.
+template <typename T, int SCH, int STG> constexpr int zwaves() {
+  return (sizeof(T) == 4 && SCH == 4 && (STG == 1 || STG == 3)) ? 4 : 1;
+}
+
 template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>()>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a) {
+__global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a) {
   constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];

@@ -46,18 +46,23 @@ def solver_name(mangled):
     return kind, prec
 
 
+def kernel_size(mangled):
+    m = re.search(r"vb::k_\w+<(?:double|float)(?:, ?(?:double|float))?, ?(\d+)", mangled)
+    return m.group(1) if m else "-"
+
+
 def stats(path, cmd):
     rows = list(csv.DictReader(open(path)))
     tot = sum(float(r["TotalDurationNs"]) for r in rows)
     out = ["# Kernel time, rocprofv3 --kernel-trace --stats", "",
            "Command (repository root):", "", "    " + cmd, "",
-           "| kernel | precision | calls | avg us | total ms | % | rocprof name |",
-           "|---|---|---:|---:|---:|---:|---|"]
+           "| kernel | n | precision | calls | avg us | total ms | % | rocprof name |",
+           "|---|---:|---|---:|---:|---:|---:|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         sn = solver_name(r["Name"]) or ("-", None)
-        out.append("| %s | %s | %s | %.1f | %.3f | %.2f | `%s` |" % (
-            sn[0], sn[1] or "-", r["Calls"], float(r["AverageNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6,
-            100 * float(r["TotalDurationNs"]) / tot, r["Name"][:90]))
+        out.append("| %s | %s | %s | %s | %.1f | %.3f | %.2f | `%s` |" % (
+            sn[0], kernel_size(r["Name"]), sn[1] or "-", r["Calls"], float(r["AverageNs"]) / 1e3,
+            float(r["TotalDurationNs"]) / 1e6, 100 * float(r["TotalDurationNs"]) / tot, r["Name"][:90]))
     return "\n".join(out) + "\n"
 
 
