@@ -22,13 +22,13 @@ constexpr int XWG = 256;  // threads per x-kernel workgroup
 
 template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
 
-// Column tile width of the y / z kernels (columns that are contiguous in
-// memory): enough columns for >= 128-byte segments and >= 256 threads.
 // Minimum waves per SIMD requested for the fused x kernels: in FP32 four
 // (the register budget that allows it costs at most a few spilled words and
 // gains more in latency hiding than it loses); FP64 keeps the full budget.
 template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
 
+// Column tile width of the y / z kernels (columns that are contiguous in
+// memory): enough columns for >= 128-byte segments and >= 256 threads.
 template <typename T, int N> constexpr int tbw() {
   constexpr int seg = 128 / int(sizeof(cpx<T>));
   constexpr int occ = 256 / TFof<N>();

@@ -181,11 +181,12 @@ struct Solver final : SolverBase {
   std::vector<hipEvent_t> evpool;
   std::map<std::string, std::pair<int, double>> ktimes;
 
-  // Fixed-dt driver iterations captured as HIP graphs.  A graph records the
-  // launches two iterations issue from one host-side pipeline state (the
-  // state index cur, the prepared/prefixed flags and dt) and is replayed only
-  // from that same state; afterwards the host state is set to what the two
-  // iterations leave behind, exactly as if they had run eagerly.
+  // Driver work captured as HIP graphs: two fixed-dt iterations, or one
+  // adaptive attempt.  A graph records the launches that work issues from one
+  // host-side pipeline state (the state index cur, the prepared/prefixed
+  // flags and dt) and is replayed only from that same state; afterwards the
+  // host state is set to what the work leaves behind, exactly as if it had
+  // run eagerly.
   struct PipeState {
     int cur;
     bool prepped, prefixed, prefixed_f;
@@ -195,13 +196,16 @@ struct Solver final : SolverBase {
              dt == o.dt;
     }
   };
+  enum { kFixedPair = 0, kAttempt = 1 };
   struct StepGraph {
     hipGraphExec_t x = nullptr;
+    int kind = kFixedPair;
     PipeState pre{}, post{};
     int evals = 0;
   };
   static constexpr int kGraphIters = 2;
-  StepGraph graphs[4];
+  static constexpr int kGraphCache = 16;   // adaptive dt visits a ladder of values
+  StepGraph graphs[kGraphCache];
   int graph_next = 0;
   PipeState fixed_post{};        // state after the last fixed iteration
   bool have_fixed_post = false;
@@ -564,22 +568,24 @@ struct Solver final : SolverBase {
     prefixed_f = q.prefixed_f;
   }
 
-  // Records kGraphIters fixed iterations from the current pipeline state.
-  // Nothing runs during the capture; the host state is put back afterwards.
-  StepGraph& capture_iters() {
+  // Records the launches of `body` (returning its evaluation count) from the
+  // current pipeline state.  Nothing runs during the capture; the host state
+  // is put back afterwards.
+  template <class F>
+  StepGraph& capture(int kind, F&& body) {
     StepGraph& G = graphs[graph_next];
-    graph_next = (graph_next + 1) % 4;
+    graph_next = (graph_next + 1) % kGraphCache;
     if (G.x) {
       VB_HIP(hipGraphExecDestroy(G.x));
       G.x = nullptr;
     }
     (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // table upload happens outside the capture
+    G.kind = kind;
     G.pre = pipe_state();
-    G.evals = 0;
     VB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipGraph_t gr = nullptr;
     try {
-      for (int i = 0; i < kGraphIters; ++i) G.evals += fixed_body();
+      G.evals = body();
     } catch (...) {
       (void)hipStreamEndCapture(st, &gr);
       if (gr) (void)hipGraphDestroy(gr);
@@ -598,10 +604,18 @@ struct Solver final : SolverBase {
     return G;
   }
 
-  StepGraph* find_graph() {
+  StepGraph& capture_iters() {
+    return capture(kFixedPair, [&] {
+      int ev = 0;
+      for (int i = 0; i < kGraphIters; ++i) ev += fixed_body();
+      return ev;
+    });
+  }
+
+  StepGraph* find_graph(int kind = kFixedPair) {
     const PipeState now = pipe_state();
     for (auto& G : graphs)
-      if (G.x && G.pre == now) return &G;
+      if (G.x && G.kind == kind && G.pre == now) return &G;
     return nullptr;
   }
 
@@ -674,13 +688,26 @@ struct Solver final : SolverBase {
       return;
     }
     if (!att_ev) VB_HIP(hipEventCreateWithFlags(&att_ev, hipEventDisableTiming));
-    const int out = eval(time_step);
     att_keep = cur;
-    cur = out;
-    prefix(FIN_PLAIN, false);
+    if (p.graphs && !prof) {
+      // one graph per pipeline state and dt; the ladder of dt values recurs
+      StepGraph* G = find_graph(kAttempt);
+      if (!G) G = &capture(kAttempt, [&] { attempt_body(); return 1; });
+      VB_HIP(hipGraphLaunch(G->x, st));
+      set_pipe_state(G->post);
+    } else {
+      attempt_body();
+    }
     VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipEventRecord(att_ev, st));
     att_pending = true;
+  }
+
+  // the launches of one adaptive attempt: evaluation and candidate energy
+  void attempt_body() {
+    const int out = eval(time_step);
+    cur = out;
+    prefix(FIN_PLAIN, false);
   }
 
   bool attempt_ready() override {

@@ -84,3 +84,17 @@ def test_batch_matches_solo(vb, gpu):
         assert got[i][1] == ref_q, i
         assert reps[i]["evaluations"] == ref_reps[0]["evaluations"], i
         assert reps[len(ps) + i]["steps"] == ref_reps[1]["steps"], i
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_adaptive_attempt_graphs_match_eager(vb, gpu, prec):
+    # adaptive attempts replay one graph per (state, flags, dt): rejections and the
+    # doubling ladder revisit dt values, so graphs are reused and re-keyed
+    p, phi = case(vb, 32, precision=prec, M1=1e4, M2=1e4, fixed_time_step=0.0, time_step=1e-6)
+    a = solo(vb, p, phi, [12, 1, 9], graphs=1)
+    b = solo(vb, p, phi, [12, 1, 9], graphs=0)
+    assert np.array_equal(a[0], b[0])
+    assert a[1] == b[1] and a[2] == b[2]
+    assert [(r["steps"], r["evaluations"], r["rejections"]) for r in a[3]] == \
+        [(r["steps"], r["evaluations"], r["rejections"]) for r in b[3]]
+    assert sum(r["rejections"] for r in a[3]) > 0      # the reject path was exercised
