@@ -149,8 +149,12 @@ struct XaArgs {
   Rows rl;
 };
 
-template <typename T, int N, bool WITHF>
-__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a) {
+// Every pipeline kernel takes its argument struct by value, or (TAB, a
+// batched launch over the solves of a size group) from a per-solve table in
+// device memory indexed by blockIdx.z.
+template <typename T, int N, bool WITHF, bool TAB = false>
+__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
+  const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
   __shared__ double red[3 * (XWG / 64)];
@@ -228,8 +232,9 @@ struct XbArgs {
   Rows rl;
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a) {
+template <typename T, int N, bool TAB = false>
+__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbArgs<T>* tab) {
+  const XbArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
@@ -374,12 +379,22 @@ __device__ __forceinline__ YIdx yidx(const Geo& g) {
   return r;
 }
 
-template <typename T, int N, int S, int TB = tbw<T, N>()>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(cpx<T>* b0, cpx<T>* b1, Geo g,
-                                                          const cpx<T>* __restrict__ tw) {
+template <typename T>
+struct YArgs {
+  cpx<T>* b0;
+  cpx<T>* b1;   // blockIdx.y = 1 transforms this one
+  const cpx<T>* tw;
+  Geo g;
+};
+
+template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(YArgs<T> a0, const YArgs<T>* tab) {
   constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
-  cpx<T>* buf = blockIdx.y ? b1 : b0;
+  const YArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  const Geo& g = a.g;
+  const cpx<T>* __restrict__ tw = a.tw;
+  cpx<T>* buf = blockIdx.y ? a.b1 : a.b0;
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   const YIdx id = yidx<TB>(g);
@@ -478,11 +493,22 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zpass(cpx<T>* buf, Geo g, 
   }
 }
 
+template <typename T>
+struct ZlArgs {
+  cpx<T>* buf;
+  const cpx<T>* tw;
+  Geo g;
+};
+
 // forward along z, multiply by -|k|^2 (laplace_kernel, Calc.cu:114-122), inverse
-template <typename T, int N, int TB = tbw<T, N>()>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(cpx<T>* buf, Geo g, const cpx<T>* __restrict__ tw) {
+template <typename T, int N, int TB = tbw<T, N>(), bool TAB = false>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(ZlArgs<T> a0, const ZlArgs<T>* tab) {
   constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
+  const ZlArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  const Geo& g = a.g;
+  const cpx<T>* __restrict__ tw = a.tw;
+  cpx<T>* buf = a.buf;
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
@@ -538,11 +564,24 @@ __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, co
   block_sum_store<1, TB * TF>(acc, red, part + blockIdx.x);
 }
 
-template <typename T, int N, int TB = tbw<T, N>()>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(const cpx<double>* __restrict__ vh, cpx<T>* P, cpx<T>* Q,
-                                                          double* part, double* dc, Geo g,
-                                                          const cpx<T>* __restrict__ tw) {
+template <typename T>
+struct ZprArgs {
+  const cpx<double>* vh;
+  cpx<T>* P;
+  cpx<T>* Q;
+  double* part;
+  double* dc;
+  const cpx<T>* tw;
+  Geo g;
+};
+
+template <typename T, int N, int TB = tbw<T, N>(), bool TAB = false>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const ZprArgs<T>* tab) {
   constexpr int E = Plan<N>::E, TF = N / E;
+  const ZprArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  const Geo& g = a.g;
+  const cpx<T>* __restrict__ tw = a.tw;
+  const cpx<double>* __restrict__ vh = a.vh;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
@@ -553,7 +592,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(const cpx<double>* _
   cpx<double> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = vh[id.q + (long long)(j + TF * e) * plane];
-  next_field_tail<T, N, TB>(x, j, id, g, lds, lds2, tw, P, Q, part, dc, red);
+  next_field_tail<T, N, TB>(x, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, a.dc, red);
 }
 
 template <typename T>
@@ -586,8 +625,10 @@ template <typename T, int SCH, int STG> constexpr int zwaves() {
   return (sizeof(T) == 4 && SCH == 4 && (STG == 1 || STG == 3)) ? 4 : 1;
 }
 
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>()>
-__global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a) {
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false>
+__global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a0,
+                                                                                  const ZsArgs<T>* tab) {
+  const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];

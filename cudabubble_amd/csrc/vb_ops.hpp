@@ -5,9 +5,44 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "vb_device.hpp"
 
 namespace vb {
+
+// Launch recording for batched launches.  While a Recorder is installed on
+// the calling thread, the pipeline launchers append each launch (its kernel,
+// its argument struct and its grid) instead of issuing it; the batched form
+// then issues one launch per recorded position for all solves of a size
+// group, with the solves' argument structs in a device table.
+enum PipeOp {
+  OP_XA, OP_XA_F, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN
+};
+struct RecLaunch {
+  // issues the table form of the recorded kernel over `lanes` solves
+  void (*launch_tab)(int op, const void* tab, dim3 grid, dim3 block, int lanes, hipStream_t s);
+  int op;
+  std::vector<unsigned char> args;
+  dim3 grid, block;
+};
+struct Recorder {
+  std::vector<RecLaunch> seq;
+};
+extern thread_local Recorder* g_recorder;
+
+template <class A>
+inline void record_launch(void (*tabfn)(int, const void*, dim3, dim3, int, hipStream_t), int op, const A& a,
+                          dim3 grid, dim3 block) {
+  RecLaunch r;
+  r.launch_tab = tabfn;
+  r.op = op;
+  r.args.assign(reinterpret_cast<const unsigned char*>(&a), reinterpret_cast<const unsigned char*>(&a) + sizeof(A));
+  r.grid = grid;
+  r.block = block;
+  g_recorder->seq.push_back(std::move(r));
+}
 
 template <typename T> struct XaArgs;
 template <typename T> struct XbArgs;

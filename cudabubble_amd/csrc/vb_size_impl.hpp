@@ -31,23 +31,29 @@ struct Launch {
     hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
   static void xa(const XaArgs<T>& a, bool withf, hipStream_t s) {
+    const dim3 gr(xgrid(a.npairs)), bl(XWG);
+    if (g_recorder) return record_launch(&launch_tab, withf ? OP_XA_F : OP_XA, a, gr, bl);
     if (withf)
-      hipLaunchKernelGGL((k_xa<T, N, true>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
+      hipLaunchKernelGGL((k_xa<T, N, true>), gr, bl, 0, s, a, nullptr);
     else
-      hipLaunchKernelGGL((k_xa<T, N, false>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
+      hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
   }
   static void xb(const XbArgs<T>& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_xb<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
+    const dim3 gr(xgrid(a.npairs)), bl(XWG);
+    if (g_recorder) return record_launch(&launch_tab, OP_XB, a, gr, bl);
+    hipLaunchKernelGGL((k_xb<T, N>), gr, bl, 0, s, a, nullptr);
   }
   static void xe(const XeArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
   static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr));
+    const dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr)), bl(YT);
+    YArgs<T> a{b0, b1, tw, g};
+    if (g_recorder) return record_launch(&launch_tab, sign < 0 ? OP_Y_FWD : OP_Y_INV, a, grid, bl);
     if (sign < 0)
-      hipLaunchKernelGGL((k_ypass<T, N, -1>), grid, dim3(YT), 0, s, b0, b1, g, tw);
+      hipLaunchKernelGGL((k_ypass<T, N, -1>), grid, bl, 0, s, a, nullptr);
     else
-      hipLaunchKernelGGL((k_ypass<T, N, +1>), grid, dim3(YT), 0, s, b0, b1, g, tw);
+      hipLaunchKernelGGL((k_ypass<T, N, +1>), grid, bl, 0, s, a, nullptr);
   }
   static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
@@ -57,24 +63,70 @@ struct Launch {
       hipLaunchKernelGGL((k_zpass<T, N, +1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_zlap<T, N>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
+    const dim3 gr(zgrid(g)), bl(YT);
+    ZlArgs<T> a{b, tw, g};
+    if (g_recorder) return record_launch(&launch_tab, OP_ZLAP, a, gr, bl);
+    hipLaunchKernelGGL((k_zlap<T, N>), gr, bl, 0, s, a, nullptr);
   }
   static void zprep(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g,
                     const cpx<T>* tw, hipStream_t s) {
-    hipLaunchKernelGGL((k_zprep<T, N>), dim3(zgrid(g)), dim3(YT), 0, s, vh, P, Q, part, dc, g, tw);
+    const dim3 gr(zgrid(g)), bl(YT);
+    ZprArgs<T> a{vh, P, Q, part, dc, tw, g};
+    if (g_recorder) return record_launch(&launch_tab, OP_ZPREP, a, gr, bl);
+    hipLaunchKernelGGL((k_zprep<T, N>), gr, bl, 0, s, a, nullptr);
   }
   static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
     const dim3 gr(zgrid(a.g)), bl(YT);
+    if (g_recorder) return record_launch(&launch_tab, sch == 4 ? OP_ZS41 + (stg - 1) : OP_ZS21 + (stg - 1), a, gr, bl);
     if (sch == 4) {
       switch (stg) {
-        case 1: hipLaunchKernelGGL((k_zstage<T, N, 4, 1>), gr, bl, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_zstage<T, N, 4, 2>), gr, bl, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_zstage<T, N, 4, 3>), gr, bl, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_zstage<T, N, 4, 4>), gr, bl, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_zstage<T, N, 4, 1>), gr, bl, 0, s, a, nullptr); break;
+        case 2: hipLaunchKernelGGL((k_zstage<T, N, 4, 2>), gr, bl, 0, s, a, nullptr); break;
+        case 3: hipLaunchKernelGGL((k_zstage<T, N, 4, 3>), gr, bl, 0, s, a, nullptr); break;
+        default: hipLaunchKernelGGL((k_zstage<T, N, 4, 4>), gr, bl, 0, s, a, nullptr); break;
       }
     } else {
-      if (stg == 1) hipLaunchKernelGGL((k_zstage<T, N, 2, 1>), gr, bl, 0, s, a);
-      else hipLaunchKernelGGL((k_zstage<T, N, 2, 2>), gr, bl, 0, s, a);
+      if (stg == 1) hipLaunchKernelGGL((k_zstage<T, N, 2, 1>), gr, bl, 0, s, a, nullptr);
+      else hipLaunchKernelGGL((k_zstage<T, N, 2, 2>), gr, bl, 0, s, a, nullptr);
+    }
+  }
+
+  // the table form of a recorded launch, over `lanes` solves (blockIdx.z)
+  static void launch_tab(int op, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
+    gr.z = unsigned(lanes);
+    switch (op) {
+      case OP_XA: hipLaunchKernelGGL((k_xa<T, N, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
+      case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
+      case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
+      case OP_Y_FWD:
+        hipLaunchKernelGGL((k_ypass<T, N, -1, TB, true>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        break;
+      case OP_Y_INV:
+        hipLaunchKernelGGL((k_ypass<T, N, +1, TB, true>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        break;
+      case OP_ZLAP: hipLaunchKernelGGL((k_zlap<T, N, TB, true>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab); break;
+      case OP_ZPREP:
+        hipLaunchKernelGGL((k_zprep<T, N, TB, true>), gr, bl, 0, s, ZprArgs<T>{}, (const ZprArgs<T>*)tab);
+        break;
+      case OP_ZS41:
+        hipLaunchKernelGGL((k_zstage<T, N, 4, 1, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      case OP_ZS42:
+        hipLaunchKernelGGL((k_zstage<T, N, 4, 2, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      case OP_ZS43:
+        hipLaunchKernelGGL((k_zstage<T, N, 4, 3, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      case OP_ZS44:
+        hipLaunchKernelGGL((k_zstage<T, N, 4, 4, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      case OP_ZS21:
+        hipLaunchKernelGGL((k_zstage<T, N, 2, 1, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      case OP_ZS22:
+        hipLaunchKernelGGL((k_zstage<T, N, 2, 2, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+        break;
+      default: break;
     }
   }
 

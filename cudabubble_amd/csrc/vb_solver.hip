@@ -35,9 +35,13 @@ struct FinArgs {
   Phys ph;
 };
 
+thread_local Recorder* g_recorder = nullptr;
+
 constexpr int kFinThreads = 1024;   // one block; wide so the partial sums load in few rounds
 
-__global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a) {
+template <bool TAB>
+__global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a0, const FinArgs* tab) {
+  const FinArgs a = TAB ? tab[blockIdx.z] : a0;
   __shared__ double red[4 * (kFinThreads / 64)];
   __shared__ double tot[4];
   double s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -94,6 +98,11 @@ __global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, 
   }
 }
 
+static void finalize_tab(int, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
+  gr.z = unsigned(lanes);
+  hipLaunchKernelGGL(k_finalize<true>, gr, bl, 0, s, FinArgs{}, (const FinArgs*)tab);
+}
+
 void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
                      const Geo& g, const Phys& ph, int mode, hipStream_t s) {
   FinArgs a;
@@ -107,7 +116,8 @@ void launch_finalize(const double* xpart, int nxb, const double* zpart, const do
   a.pad = 0;
   a.g = g;
   a.ph = ph;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, a);
+  if (g_recorder) return record_launch(&finalize_tab, OP_FIN, a, dim3(1), dim3(kFinThreads));
+  hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(kFinThreads), 0, s, a, nullptr);
 }
 
 hipStream_t make_queue_stream() {
@@ -654,6 +664,57 @@ struct Solver final : SolverBase {
   }
 
   bool fixed_dt() const override { return p.fixed_time_step > 0 && p.scheme != VB_EULER; }
+
+  // ---- batched launches (see SolverBase) -------------------------------------
+  PipeState rec_pre{}, rec_post{};
+  int rec_evals = 0;
+
+  GroupKey group_key() const override {
+    GroupKey k{};
+    k.ok = (fixed_dt() && p.graphs) ? 1 : 0;
+    k.nx = p.nx;
+    k.nz = p.nz;
+    k.precision = p.precision;
+    k.scheme = p.scheme;
+    k.drift = drift_mode() ? 1 : 0;
+    return k;
+  }
+
+  int record_pair(Recorder& rec) override {
+    rec_evals = 0;
+    if (!started || prof || !have_fixed_post || !(fixed_post == pipe_state())) return 0;
+    (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // tables exist before recording
+    rec_pre = pipe_state();
+    g_recorder = &rec;
+    int ev = 0;
+    try {
+      for (int i = 0; i < kGraphIters; ++i) ev += fixed_body();
+    } catch (...) {
+      g_recorder = nullptr;
+      set_pipe_state(rec_pre);
+      throw;
+    }
+    g_recorder = nullptr;
+    rec_post = pipe_state();
+    set_pipe_state(rec_pre);
+    rec_evals = ev;
+    return ev;
+  }
+
+  bool at_recorded_state() const override { return rec_evals > 0 && pipe_state() == rec_pre; }
+
+  void advance_recorded(vb_step_report* r) override {
+    cur_E_known = false;
+    set_pipe_state(rec_post);
+    fixed_post = rec_post;
+    have_fixed_post = true;
+    r->evaluations += rec_evals;
+    for (int i = 0; i < kGraphIters; ++i) {
+      time_current += time_step;
+      steps++;
+      r->steps++;
+    }
+  }
 
   // ---- adaptive ETD iterations in two halves (belastic.cpp:639-669, 713-746)
   // adaptive_issue() launches one attempt: the evaluation at the current dt,

@@ -12,6 +12,19 @@
 
 namespace vb {
 
+// Solves whose pipelines issue the same launches (grid and kernel) and can
+// therefore share batched launches.
+struct GroupKey {
+  int ok;   // 0: never grouped (adaptive dt, Euler, graphs off)
+  int nx, nz, precision, scheme, drift;
+  bool operator<(const GroupKey& o) const {
+    const int a[6] = {ok, nx, nz, precision, scheme, drift}, b[6] = {o.ok, o.nx, o.nz, o.precision, o.scheme, o.drift};
+    for (int i = 0; i < 6; ++i)
+      if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+  }
+};
+
 struct SolverBase {
   vb_params p{};
   virtual ~SolverBase() = default;
@@ -38,6 +51,16 @@ struct SolverBase {
   // attempt_ready() polls it; adaptive_complete() accepts it (returns 1, time
   // and steps advanced, exits in rep) or rejects it (returns 0, dt halved).
   // step(n) on an adaptive solver is issue/complete until n acceptances.
+  // Batched launches over the solves of a size group (fixed dt): record the
+  // launches of two driver iterations from the current state instead of
+  // issuing them (returns their evaluations, 0 if the state does not recur
+  // yet); at_recorded_state() tells whether a replay of that recording is
+  // valid now; advance_recorded() moves the host state as the two iterations
+  // would.
+  virtual GroupKey group_key() const = 0;
+  virtual int record_pair(Recorder& rec) = 0;
+  virtual bool at_recorded_state() const = 0;
+  virtual void advance_recorded(vb_step_report* rep) = 0;
   virtual void adaptive_issue(vb_step_report& rep) = 0;
   virtual bool attempt_ready() = 0;
   virtual int adaptive_complete(vb_step_report& rep) = 0;

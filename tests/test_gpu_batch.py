@@ -98,3 +98,34 @@ def test_adaptive_attempt_graphs_match_eager(vb, gpu, prec):
     assert [(r["steps"], r["evaluations"], r["rejections"]) for r in a[3]] == \
         [(r["steps"], r["evaluations"], r["rejections"]) for r in b[3]]
     assert sum(r["rejections"] for r in a[3]) > 0      # the reject path was exercised
+
+
+def test_size_group_launches_match_solo(vb, gpu):
+    # solves of one size group share every launch (grid.z over the solves);
+    # each must still compute exactly what it computes alone
+    specs = []
+    for j, (e, m1, m2) in enumerate([(1.5, 1e3, 2e4), (2.0, 1e4, 1e4), (3.0, 3e4, 1e3), (2.0, 5e3, 5e4)]):
+        specs.append(dict(nx=32, kind=(1, 3, 4, 1)[j], seed=60 + j, precision=1, M1=m1, M2=m2,
+                          eps=e * 2 * np.pi / 32, phi_lut_mode=1, time_step=2e-7, fixed_time_step=2e-7))
+    for j in range(3):
+        specs.append(dict(nx=48, nz=32, kind=2, seed=70 + j, precision=0, M1=1e4 * (j + 1), M2=1e4))
+    for j in range(2):
+        specs.append(dict(nx=32, kind=1, seed=80 + j, precision=1, M1=0.0, M2=0.0))
+    ps, phis = [], []
+    for sp in specs:
+        sp = dict(sp)
+        p, phi = case(vb, sp.pop("nx"), sp.pop("nz", None), sp.pop("kind"), sp.pop("seed"), **sp)
+        ps.append(p)
+        phis.append(phi)
+    flush_print("grouped batch of", len(ps))
+    b = vb.Batch(ps, phis)
+    reps = b.step(7) + b.step(6)
+    got = [(b.get_phi(i), b.query(i), b.report(i)) for i in range(len(ps))]
+    b.close()
+    for i, (p, phi) in enumerate(zip(ps, phis)):
+        assert np.isfinite(got[i][0]).all() and reps[i]["status"] == 0, i
+        ref_phi, ref_state, ref_q, ref_reps = solo(vb, p, phi, [7, 6], graphs=0)
+        assert np.array_equal(got[i][0], ref_phi), i
+        assert got[i][1] == ref_q, i
+        assert got[i][2]["time_current"] == ref_state["time_current"], i
+        assert reps[i]["evaluations"] == ref_reps[0]["evaluations"], i
