@@ -116,9 +116,7 @@ struct vb_server {
       const double e0 = r.hist.front(), e1 = r.hist.back();
       if (fabs(e0 - e1) <= cfg.converge_tol * fabs(e0)) return true;
     }
-    double t;
-    r.s->state(nullptr, nullptr, nullptr, &t);
-    if (r.req.p.tmax > 0 && t >= r.req.p.tmax) return true;
+    if (r.req.p.tmax > 0 && r.rep.time_current >= r.req.p.tmax) return true;
     if (cfg.max_steps > 0 && r.rep.steps >= cfg.max_steps) return true;
     return false;
   }

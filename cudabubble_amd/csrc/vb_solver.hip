@@ -805,6 +805,8 @@ struct Solver final : SolverBase {
     time_current += time_step;   // belastic.cpp:742-746
     steps++;
     r.steps++;
+    r.time_step = time_step;
+    r.time_current = time_current;
     return 1;
   }
 
@@ -1034,7 +1036,7 @@ struct Solver final : SolverBase {
       ensure_prefixed();
       read_scal();
     } else {
-      if (started && p.penalty_drift) sync_targets_from_device();
+      if (targets_drift()) sync_targets_from_device();
       integrals64();
     }
     if (E) *E = hs->E;
@@ -1043,8 +1045,11 @@ struct Solver final : SolverBase {
     if (Eb) *Eb = hs->Eb;
   }
 
+  // the targets move on the device only under the fixed-dt penalty drift
+  bool targets_drift() const { return started && fixed_dt() && drift_mode(); }
+
   void state(double* a, double* b, double* dt, double* t) override {
-    if (started && p.penalty_drift) sync_targets_from_device();
+    if (targets_drift()) sync_targets_from_device();
     if (a) *a = alpha;
     if (b) *b = beta;
     if (dt) *dt = time_step;
