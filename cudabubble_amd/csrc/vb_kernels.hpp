@@ -18,7 +18,7 @@
 
 namespace vb {
 
-constexpr int XWG = 256;  // threads per x-kernel workgroup
+constexpr int XWG = 64;   // threads per x-kernel workgroup: one wave (measured best at 256^3: 64 < 128 < 256)
 
 template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
 
