@@ -46,24 +46,29 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a) {
 // ---------------------------------------------------------------------------
 // radix plans: E elements per thread, TF threads per transform, radices
 // ---------------------------------------------------------------------------
-template <int N> struct Plan;
-template <> struct Plan<16>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
-template <> struct Plan<32>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 4}; };
-template <> struct Plan<48>  { static constexpr int E = 12, NP = 3; static constexpr int R[3] = {4, 4, 3}; };
-template <> struct Plan<64>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 8}; };
-template <> struct Plan<96>  { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 2, 3}; };
-template <> struct Plan<128> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 4, 4}; };
-template <> struct Plan<192> { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 4, 3}; };
-template <> struct Plan<256> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 4}; };
-template <> struct Plan<384> { static constexpr int E = 12, NP = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
-template <> struct Plan<512> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+// V = 0 is the plan of every kernel.  V = 1 trades registers for fewer
+// Stockham passes (fewer LDS exchanges and workgroup barriers); it differs
+// only where measured to pay: 256 = 16 * 16 for the FP32 column kernels.
+template <int N, int V = 0> struct Plan;
+template <> struct Plan<16, 0>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
+template <> struct Plan<32, 0>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 4}; };
+template <> struct Plan<48, 0>  { static constexpr int E = 12, NP = 3; static constexpr int R[3] = {4, 4, 3}; };
+template <> struct Plan<64, 0>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 8}; };
+template <> struct Plan<96, 0>  { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 2, 3}; };
+template <> struct Plan<128, 0> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 4, 4}; };
+template <> struct Plan<192, 0> { static constexpr int E = 12, NP = 4; static constexpr int R[4] = {4, 4, 4, 3}; };
+template <> struct Plan<256, 0> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 4}; };
+template <> struct Plan<384, 0> { static constexpr int E = 12, NP = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
+template <> struct Plan<512, 0> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+template <int N> struct Plan<N, 1> : Plan<N, 0> {};
+template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 16}; };
 
-template <int N> constexpr int TFof() { return N / Plan<N>::E; }
+template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 
-template <int N, int P>
+template <int N, int P, int V>
 constexpr int ns_before() {
   int ns = 1;
-  for (int i = 0; i < P; ++i) ns *= Plan<N>::R[i];
+  for (int i = 0; i < P; ++i) ns *= Plan<N, V>::R[i];
   return ns;
 }
 
@@ -119,19 +124,58 @@ template <int S, typename T> struct Dft<8, S, T> {
   }
 };
 
+// 16 points as 4 x 4: n = n2 + 4 n1, k = k1 + 4 k2, so
+// X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[n2 + 4 n1] W4^(n1 k1).
+template <int S, typename T> struct Dft<16, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    cpx<T> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cpx<T> t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<4, S, T>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+    // W16^e = cos(2 pi e / 16) + S i sin(2 pi e / 16) for e = n2 k1 in {1, 2, 3, 4, 6, 9}
+    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173), c2 = T(0.70710678118654752440);
+#pragma unroll
+    for (int n2 = 1; n2 < 4; ++n2) {
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) {
+        const int e = n2 * k1;
+        T c, sn;
+        if (e == 1) { c = c1; sn = s1; }
+        else if (e == 2) { c = c2; sn = c2; }
+        else if (e == 3) { c = s1; sn = c1; }
+        else if (e == 4) { c = T(0); sn = T(1); }
+        else if (e == 6) { c = -c2; sn = c2; }
+        else { c = -c1; sn = -s1; }   // e = 9
+        a[n2][k1] = cmul(a[n2][k1], mk<T>(c, T(S) * sn));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cpx<T> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Dft<4, S, T>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Stockham passes.  Lds is a functor: lds(pos) -> cpx<T>& for this lane.
 // ---------------------------------------------------------------------------
 // Butterflies of pass P for one transform held in x.  Non-last passes write
 // their outputs to LDS; the last pass leaves them in y (same positions).
-template <typename T, int N, int S, int P, class Lds>
+template <typename T, int N, int S, int P, int V, class Lds>
 __device__ __forceinline__ void pass_body(const cpx<T>* x, cpx<T>* y, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
-  constexpr int E = Plan<N>::E;
+  constexpr int E = Plan<N, V>::E;
   constexpr int TF = N / E;
-  constexpr int R = Plan<N>::R[P];
-  constexpr int NS = ns_before<N, P>();
+  constexpr int R = Plan<N, V>::R[P];
+  constexpr int NS = ns_before<N, P, V>();
   constexpr int ER = E / R;
-  constexpr bool last = (P == Plan<N>::NP - 1);
+  constexpr bool last = (P == Plan<N, V>::NP - 1);
   constexpr int TST = N / (NS * R);
 #pragma unroll
   for (int t = 0; t < ER; ++t) {
@@ -160,15 +204,15 @@ __device__ __forceinline__ void pass_body(const cpx<T>* x, cpx<T>* y, int j, con
   }
 }
 
-template <typename T, int N, int S, int P>
+template <typename T, int N, int S, int P, int V>
 struct Pass {
   template <class Lds>
   static __device__ __forceinline__ void run(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
-    constexpr int E = Plan<N>::E;
+    constexpr int E = Plan<N, V>::E;
     constexpr int TF = N / E;
-    constexpr bool last = (P == Plan<N>::NP - 1);
+    constexpr bool last = (P == Plan<N, V>::NP - 1);
     cpx<T> y[last ? E : 1];
-    pass_body<T, N, S, P>(x, y, j, tw, lds);
+    pass_body<T, N, S, P, V>(x, y, j, tw, lds);
     if constexpr (last) {
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = y[e];
@@ -177,24 +221,24 @@ struct Pass {
 #pragma unroll
       for (int e = 0; e < E; ++e) x[e] = lds(j + TF * e);
       __syncthreads();
-      Pass<T, N, S, P + 1>::run(x, j, tw, lds);
+      Pass<T, N, S, P + 1, V>::run(x, j, tw, lds);
     }
   }
 };
 
 // Two independent transforms through the same passes: one set of barriers
 // for both, and twice the independent work between barriers.
-template <typename T, int N, int S, int P>
+template <typename T, int N, int S, int P, int V>
 struct Pass2 {
   template <class Lds>
   static __device__ __forceinline__ void run(cpx<T>* x0, cpx<T>* x1, int j, const cpx<T>* __restrict__ tw,
                                              Lds& l0, Lds& l1) {
-    constexpr int E = Plan<N>::E;
+    constexpr int E = Plan<N, V>::E;
     constexpr int TF = N / E;
-    constexpr bool last = (P == Plan<N>::NP - 1);
+    constexpr bool last = (P == Plan<N, V>::NP - 1);
     cpx<T> y0[last ? E : 1], y1[last ? E : 1];
-    pass_body<T, N, S, P>(x0, y0, j, tw, l0);
-    pass_body<T, N, S, P>(x1, y1, j, tw, l1);
+    pass_body<T, N, S, P, V>(x0, y0, j, tw, l0);
+    pass_body<T, N, S, P, V>(x1, y1, j, tw, l1);
     if constexpr (last) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -209,7 +253,7 @@ struct Pass2 {
         x1[e] = l1(j + TF * e);
       }
       __syncthreads();
-      Pass2<T, N, S, P + 1>::run(x0, x1, j, tw, l0, l1);
+      Pass2<T, N, S, P + 1, V>::run(x0, x1, j, tw, l0, l1);
     }
   }
 };
@@ -217,16 +261,16 @@ struct Pass2 {
 // Full 1-D transform (unnormalised): S = -1 forward, +1 inverse.
 // Contains (NP - 1) * 2 workgroup barriers; every thread of the workgroup
 // must call it the same number of times.
-template <typename T, int N, int S, class Lds>
+template <typename T, int N, int S, int V = 0, class Lds>
 __device__ __forceinline__ void fft(cpx<T>* x, int j, const cpx<T>* __restrict__ tw, Lds& lds) {
-  Pass<T, N, S, 0>::run(x, j, tw, lds);
+  Pass<T, N, S, 0, V>::run(x, j, tw, lds);
 }
 
 // Two transforms of the same length and direction, each with its own LDS view.
-template <typename T, int N, int S, class Lds>
+template <typename T, int N, int S, int V = 0, class Lds>
 __device__ __forceinline__ void fft2(cpx<T>* x0, cpx<T>* x1, int j, const cpx<T>* __restrict__ tw, Lds& l0,
                                      Lds& l1) {
-  Pass2<T, N, S, 0>::run(x0, x1, j, tw, l0, l1);
+  Pass2<T, N, S, 0, V>::run(x0, x1, j, tw, l0, l1);
 }
 
 // LDS view for a lane whose transform is stored contiguously (x kernels):

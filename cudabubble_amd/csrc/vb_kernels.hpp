@@ -27,6 +27,10 @@ template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
 // gains more in latency hiding than it loses); FP64 keeps the full budget.
 template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
 
+// Plan variant of the FP32 column kernels (y passes, z Laplacian, stage-3
+// combine) at 256: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
+template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && N == 256) ? 1 : 0; }
+
 // Column tile width of the y / z kernels (columns that are contiguous in
 // memory): enough columns for >= 128-byte segments and >= 256 threads.
 template <typename T, int N> constexpr int tbw() {
@@ -387,9 +391,9 @@ struct YArgs {
   Geo g;
 };
 
-template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(YArgs<T> a0, const YArgs<T>* tab) {
-  constexpr int E = Plan<N>::E, TF = N / E;
+template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false, int V = 0>
+__global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, const YArgs<T>* tab) {
+  constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const YArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   const Geo& g = a.g;
@@ -402,7 +406,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_ypass(YArgs<T> a0, const Y
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = col[(long long)(j + TF * e) * g.pitch];
-  fft<T, N, S>(x, j, tw, lds);
+  fft<T, N, S, V>(x, j, tw, lds);
   if (id.valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
@@ -501,9 +505,9 @@ struct ZlArgs {
 };
 
 // forward along z, multiply by -|k|^2 (laplace_kernel, Calc.cu:114-122), inverse
-template <typename T, int N, int TB = tbw<T, N>(), bool TAB = false>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(ZlArgs<T> a0, const ZlArgs<T>* tab) {
-  constexpr int E = Plan<N>::E, TF = N / E;
+template <typename T, int N, int TB = tbw<T, N>(), bool TAB = false, int V = 0>
+__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zlap(ZlArgs<T> a0, const ZlArgs<T>* tab) {
+  constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const ZlArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   const Geo& g = a.g;
@@ -517,14 +521,14 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(ZlArgs<T> a0, const Z
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = col[(long long)(j + TF * e) * plane];
-  fft<T, N, -1>(x, j, tw, lds);
+  fft<T, N, -1, V>(x, j, tw, lds);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
     const T mult = T(-double(mindex(g, id.kx, id.ky, kz)) * g.kfac2);
     x[e] = scale(x[e], mult);
   }
-  fft<T, N, +1>(x, j, tw, lds);
+  fft<T, N, +1, V>(x, j, tw, lds);
   if (id.valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * plane] = x[e];
@@ -535,11 +539,11 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zlap(ZlArgs<T> a0, const Z
 // state and the combine stay FP64 in both precisions): its Parseval partial
 // of |grad|^2 and its mean (the k = 0 coefficient, written to dc[0]) are taken
 // in FP64; then P = z-inverse(nf), Q = z-inverse(-|k|^2 nf) in T.
-template <typename T, int N, int TB, class Lds>
+template <typename T, int N, int TB, int V, class Lds>
 __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, const ZIdx& id, const Geo& g, Lds& lds,
                                                 Lds& lds2, const cpx<T>* __restrict__ tw, cpx<T>* P, cpx<T>* Q,
                                                 double* part, double* dc, double* red) {
-  constexpr int E = Plan<N>::E, TF = N / E;
+  constexpr int E = Plan<N, V>::E, TF = N / E;
   const long long plane = g.pp;
   double acc[1] = {0.0};
   cpx<T> x[E], y[E];
@@ -553,7 +557,7 @@ __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, co
     y[e] = mk<T>(T(re * m), T(im * m));
   }
   if (id.valid && id.q == 0 && j == 0) dc[0] = nf[0].x;
-  fft2<T, N, +1>(x, y, j, tw, lds, lds2);
+  fft2<T, N, +1, V>(x, y, j, tw, lds, lds2);
   if (id.valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -592,7 +596,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
   cpx<double> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = vh[id.q + (long long)(j + TF * e) * plane];
-  next_field_tail<T, N, TB>(x, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, a.dc, red);
+  next_field_tail<T, N, TB, 0>(x, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, a.dc, red);
 }
 
 template <typename T>
@@ -622,14 +626,19 @@ struct ZsArgs {
 // and 3 fit four waves with a few spilled words and run faster so; stages 2
 // and 4 (FP64 state and two stage spectra live) would spill heavily.
 template <typename T, int SCH, int STG> constexpr int zwaves() {
-  return (sizeof(T) == 4 && SCH == 4 && (STG == 1 || STG == 3)) ? 4 : 1;
+  return (sizeof(T) == 4 && SCH == 4 && STG == 1) ? 4 : 1;
+}
+// plan variant of a stage combine: 16 x 16 pays for stage 3 at 256 (FP32)
+template <typename T, int N, int SCH, int STG> constexpr int zsplan() {
+  return (SCH == 4 && STG == 3) ? colplan<T, N>() : 0;
 }
 
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false>
-__global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a0,
-                                                                                  const ZsArgs<T>* tab) {
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false,
+          int V = zsplan<T, N, SCH, STG>()>
+__global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a0,
+                                                                                     const ZsArgs<T>* tab) {
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int E = Plan<N>::E, TF = N / E;
+  constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
@@ -653,7 +662,7 @@ __global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_z
     if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[i]; pb[e] = a.Nb[i]; }
     if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[i];
   }
-  fft<T, N, -1>(x, j, a.tw, lds);   // x = N^ of this stage
+  fft<T, N, -1, V>(x, j, a.tw, lds);   // x = N^ of this stage
   cpx<double> nf[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -707,7 +716,7 @@ __global__ void __launch_bounds__((TB * TFof<N>()), (zwaves<T, SCH, STG>())) k_z
     }
     nf[e] = mk<double>(fr, fi);
   }
-  next_field_tail<T, N, TB>(nf, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, a.dc, red);
+  next_field_tail<T, N, TB, V>(nf, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, a.dc, red);
 }
 
 // widening copy of a spectrum into the FP64 state

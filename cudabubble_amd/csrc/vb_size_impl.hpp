@@ -12,6 +12,9 @@ struct Launch {
   static constexpr int G = xlanes<N>();
   static constexpr int TB = tbw<T, N>();
   static constexpr int YT = TB * TF;
+  static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
+  static constexpr int CT = TB * TFof<N, CV>();       // their threads per workgroup
+  template <int SCH, int STG> static constexpr int zs_threads() { return TB * TFof<N, zsplan<T, N, SCH, STG>()>(); }
 
   static unsigned xgrid(long long npairs) { return unsigned((npairs + G - 1) / G); }
 
@@ -47,13 +50,13 @@ struct Launch {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
   static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr)), bl(YT);
+    const dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr)), bl(CT);
     YArgs<T> a{b0, b1, tw, g};
     if (g_recorder) return record_launch(&launch_tab, sign < 0 ? OP_Y_FWD : OP_Y_INV, a, grid, bl);
     if (sign < 0)
-      hipLaunchKernelGGL((k_ypass<T, N, -1>), grid, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_ypass<T, N, -1, TB, false, CV>), grid, bl, 0, s, a, nullptr);
     else
-      hipLaunchKernelGGL((k_ypass<T, N, +1>), grid, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_ypass<T, N, +1, TB, false, CV>), grid, bl, 0, s, a, nullptr);
   }
   static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
@@ -63,10 +66,10 @@ struct Launch {
       hipLaunchKernelGGL((k_zpass<T, N, +1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const dim3 gr(zgrid(g)), bl(YT);
+    const dim3 gr(zgrid(g)), bl(CT);
     ZlArgs<T> a{b, tw, g};
     if (g_recorder) return record_launch(&launch_tab, OP_ZLAP, a, gr, bl);
-    hipLaunchKernelGGL((k_zlap<T, N>), gr, bl, 0, s, a, nullptr);
+    hipLaunchKernelGGL((k_zlap<T, N, TB, false, CV>), gr, bl, 0, s, a, nullptr);
   }
   static void zprep(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g,
                     const cpx<T>* tw, hipStream_t s) {
@@ -76,7 +79,12 @@ struct Launch {
     hipLaunchKernelGGL((k_zprep<T, N>), gr, bl, 0, s, a, nullptr);
   }
   static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
-    const dim3 gr(zgrid(a.g)), bl(YT);
+    const dim3 gr(zgrid(a.g));
+    int nt = YT;
+    if (sch == 4) nt = stg == 1 ? zs_threads<4, 1>() : stg == 2 ? zs_threads<4, 2>() : stg == 3 ? zs_threads<4, 3>()
+                                                                                           : zs_threads<4, 4>();
+    else nt = stg == 1 ? zs_threads<2, 1>() : zs_threads<2, 2>();
+    const dim3 bl = dim3((unsigned)nt);
     if (g_recorder) return record_launch(&launch_tab, sch == 4 ? OP_ZS41 + (stg - 1) : OP_ZS21 + (stg - 1), a, gr, bl);
     if (sch == 4) {
       switch (stg) {
@@ -99,12 +107,14 @@ struct Launch {
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
       case OP_Y_FWD:
-        hipLaunchKernelGGL((k_ypass<T, N, -1, TB, true>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        hipLaunchKernelGGL((k_ypass<T, N, -1, TB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
       case OP_Y_INV:
-        hipLaunchKernelGGL((k_ypass<T, N, +1, TB, true>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        hipLaunchKernelGGL((k_ypass<T, N, +1, TB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
-      case OP_ZLAP: hipLaunchKernelGGL((k_zlap<T, N, TB, true>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab); break;
+      case OP_ZLAP:
+        hipLaunchKernelGGL((k_zlap<T, N, TB, true, CV>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab);
+        break;
       case OP_ZPREP:
         hipLaunchKernelGGL((k_zprep<T, N, TB, true>), gr, bl, 0, s, ZprArgs<T>{}, (const ZprArgs<T>*)tab);
         break;
