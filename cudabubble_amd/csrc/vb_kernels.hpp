@@ -463,6 +463,13 @@ __host__ __device__ inline long long zgroups(const Geo& g, int TB) {
   return ((long long)g.ny * g.nh + TB - 1) / TB;
 }
 
+// Elements of a stage-private spectrum (pre, Na, Nb of k_zstage): group-major,
+// [z workgroup][kz][TB], which may exceed the padded spectrum for unaligned rows.
+__host__ __device__ inline long long zs_private(const Geo& g, int TB) {
+  const long long n = zgroups(g, TB) * g.nz * TB;
+  return n > g.ncp ? n : g.ncp;
+}
+
 // |k|^2 of the gradient with each component's own Nyquist plane zeroed
 // (gradient_kernel, Calc.cu:310-339), times the half-spectrum weight.
 __device__ __forceinline__ double grad_w(const Geo& g, int kx, int ky, int kz) {
@@ -535,28 +542,26 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zlap(ZlArgs<T> a0, cons
   }
 }
 
-// Shared tail of zprep / zstage.  nf is the next stage field in FP64 (the
-// state and the combine stay FP64 in both precisions): its Parseval partial
-// of |grad|^2 and its mean (the k = 0 coefficient, written to dc[0]) are taken
-// in FP64; then P = z-inverse(nf), Q = z-inverse(-|k|^2 nf) in T.
+// Shared tail of zprep / zstage, applied to the next stage field nf in FP64
+// (the state and the combine stay FP64 in both precisions), one element at a
+// time as the combine produces it: its Parseval partial of |grad|^2 and its
+// mean (the k = 0 coefficient, written to dc[0]) are taken in FP64; then
+// P = z-inverse(nf), Q = z-inverse(-|k|^2 nf) in T.
+template <typename T>
+__device__ __forceinline__ void next_field_elem(double re, double im, int kz, const ZIdx& id, const Geo& g,
+                                                double& acc, cpx<T>& xo, cpx<T>& yo) {
+  if (id.valid) acc += grad_w(g, id.kx, id.ky, kz) * (re * re + im * im);
+  const double m = -double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
+  xo = mk<T>(T(re), T(im));
+  yo = mk<T>(T(re * m), T(im * m));
+}
+
 template <typename T, int N, int TB, int V, class Lds>
-__device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, const ZIdx& id, const Geo& g, Lds& lds,
-                                                Lds& lds2, const cpx<T>* __restrict__ tw, cpx<T>* P, cpx<T>* Q,
-                                                double* part, double* dc, double* red) {
+__device__ __forceinline__ void next_field_tail(cpx<T>* x, cpx<T>* y, double acc, int j, const ZIdx& id,
+                                                const Geo& g, Lds& lds, Lds& lds2, const cpx<T>* __restrict__ tw,
+                                                cpx<T>* P, cpx<T>* Q, double* part, double* red) {
   constexpr int E = Plan<N, V>::E, TF = N / E;
   const long long plane = g.pp;
-  double acc[1] = {0.0};
-  cpx<T> x[E], y[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int kz = j + TF * e;
-    const double re = nf[e].x, im = nf[e].y;
-    if (id.valid) acc[0] += grad_w(g, id.kx, id.ky, kz) * (re * re + im * im);
-    const double m = -double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
-    x[e] = mk<T>(T(re), T(im));
-    y[e] = mk<T>(T(re * m), T(im * m));
-  }
-  if (id.valid && id.q == 0 && j == 0) dc[0] = nf[0].x;
   fft2<T, N, +1, V>(x, y, j, tw, lds, lds2);
   if (id.valid) {
 #pragma unroll
@@ -565,7 +570,8 @@ __device__ __forceinline__ void next_field_tail(const cpx<double>* nf, int j, co
       Q[id.q + (long long)(j + TF * e) * plane] = y[e];
     }
   }
-  block_sum_store<1, TB * TF>(acc, red, part + blockIdx.x);
+  double s[1] = {acc};
+  block_sum_store<1, TB * TF>(s, red, part + blockIdx.x);
 }
 
 template <typename T>
@@ -593,10 +599,15 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
   ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
   const long long plane = g.pp;
-  cpx<double> x[E];
+  cpx<double> u[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = vh[id.q + (long long)(j + TF * e) * plane];
-  next_field_tail<T, N, TB, 0>(x, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, a.dc, red);
+  for (int e = 0; e < E; ++e) u[e] = vh[id.q + (long long)(j + TF * e) * plane];
+  if (id.valid && id.q == 0 && j == 0) a.dc[0] = u[0].x;
+  cpx<T> x[E], y[E];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) next_field_elem<T>(u[e].x, u[e].y, j + TF * e, id, g, acc, x[e], y[e]);
+  next_field_tail<T, N, TB, 0>(x, y, acc, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, red);
 }
 
 template <typename T>
@@ -628,6 +639,9 @@ struct ZsArgs {
 template <typename T, int SCH, int STG> constexpr int zwaves() {
   return (sizeof(T) == 4 && SCH == 4 && STG == 1) ? 4 : 1;
 }
+// whether a stage combine loads its other operands before its forward
+// transform (overlapping them with it) or after it (fewer live registers)
+template <typename T, int SCH, int STG> constexpr bool zprefetch() { return true; }
 // plan variant of a stage combine: 16 x 16 pays for stage 3 at 256 (FP32)
 template <typename T, int N, int SCH, int STG> constexpr int zsplan() {
   return (SCH == 4 && STG == 3) ? colplan<T, N>() : 0;
@@ -647,6 +661,11 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g);
   const long long plane = g.pp;
+  // pre, Na and Nb live only between the stages of one evaluation, so they
+  // are kept group-major: this workgroup's TB columns of all N planes are one
+  // contiguous run (see zs_private), which keeps the strided z traffic of a
+  // stage to the spectra that other kernels share
+  const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
@@ -654,20 +673,27 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
   cpx<double> ur[USES_U ? E : 1];
   cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? E : 1];
   cpx<T> pb[(SCH == 4 && STG == 4) ? E : 1];
+  auto load_operands = [&]() {
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const long long i = id.q + (long long)(j + TF * e) * plane;
-    if constexpr (USES_U) ur[e] = a.uh[i];
-    if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[i];
-    if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[i]; pb[e] = a.Nb[i]; }
-    if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[i];
-  }
+    for (int e = 0; e < E; ++e) {
+      const long long i = id.q + (long long)(j + TF * e) * plane;
+      const long long ib = ib0 + (long long)(j + TF * e) * TB;
+      if constexpr (USES_U) ur[e] = a.uh[i];
+      if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[ib];
+      if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[ib]; pb[e] = a.Nb[ib]; }
+      if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[ib];
+    }
+  };
+  if constexpr (zprefetch<T, SCH, STG>()) load_operands();
   fft<T, N, -1, V>(x, j, a.tw, lds);   // x = N^ of this stage
-  cpx<double> nf[E];
+  if constexpr (!zprefetch<T, SCH, STG>()) load_operands();
+  cpx<T> y[E];
+  double acc = 0.0, dc0 = 0.0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
     const long long i = id.q + (long long)kz * plane;
+    const long long ib = ib0 + (long long)kz * TB;
     const int m = mindex(g, id.kx, id.ky, kz);
     const double nr = double(x[e].x), ni = double(x[e].y);
     double fr, fi;   // next field
@@ -676,19 +702,19 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
       const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd41: a^
       fi = c.x * u.y + c.y * ni;
-      if (id.valid) a.pre[i] = mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni));
+      if (id.valid) a.pre[ib] = mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni));
     } else if constexpr (SCH == 4 && STG == 2) {
       const double2 c = a.c1[m];
       const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd42: b^
       fi = c.x * u.y + c.y * ni;
-      if (id.valid) a.Na[i] = x[e];
+      if (id.valid) a.Na[ib] = x[e];
     } else if constexpr (SCH == 4 && STG == 3) {
       const double q = a.c1[m].y;
       const cpx<T> p = pa[e];
       fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
       fi = double(p.y) + 2.0 * q * ni;
-      if (id.valid) a.Nb[i] = x[e];
+      if (id.valid) a.Nb[ib] = x[e];
     } else if constexpr (SCH == 4 && STG == 4) {
       const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
       const cpx<double> u = ur[e];
@@ -702,7 +728,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
       const cpx<double> u = ur[e];
       fr = c.x * u.x + c.y * nr;                      // etd1: a^
       fi = c.x * u.y + c.y * ni;
-      if (id.valid) a.Na[i] = x[e];
+      if (id.valid) a.Na[ib] = x[e];
     } else {                                          // SCH == 2 && STG == 2
       const double2 c = a.c1[m];
       const double c2 = a.c2[m].x;
@@ -714,9 +740,11 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
       fi = ai + c2 * (ni - double(nu.y));
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
     }
-    nf[e] = mk<double>(fr, fi);
+    if (e == 0) dc0 = fr;
+    next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
   }
-  next_field_tail<T, N, TB, V>(nf, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, a.dc, red);
+  if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
+  next_field_tail<T, N, TB, V>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
 // widening copy of a spectrum into the FP64 state

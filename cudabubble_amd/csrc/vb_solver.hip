@@ -253,7 +253,8 @@ struct Solver final : SolverBase {
     }
     const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
     for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
-    for (auto* b : {&P, &Q, &Tt, &pre, &Na, &Nb}) b->alloc(nc);
+    for (auto* b : {&P, &Q, &Tt}) b->alloc(nc);
+    for (auto* b : {&pre, &Na, &Nb}) b->alloc((size_t)zs_private(g, oz->ytile));
     if (p.scheme == VB_EULER || p.shape_adjust) W.alloc(nc);
     dc.alloc(1);
     v.alloc(nr);
