@@ -39,7 +39,7 @@ def alg_bytes(name, n, prec_bytes):
     C = modes * cs
     R = n ** 3 * prec_bytes
     table = {
-        "y_inv2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_fwd": 2 * C, "z_inv": 2 * C,
+        "y_inv2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_lap_fin": 2 * C, "z_fwd": 2 * C, "z_inv": 2 * C,
         "xa": 3 * C, "xa_f": 4 * C + 2 * R, "xb": 4 * C,
         "zprep": modes * (16 + 2 * cs),
         # stage combines: N spectrum in, stage/state spectra, next-stage P, Q out

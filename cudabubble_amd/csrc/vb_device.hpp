@@ -79,4 +79,114 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double* red, d
   }
 }
 
+// Workgroup shares of a split evaluation of the integrals and the ticket that
+// picks the workgroup which adds them; one per solve, zeroed at creation and
+// left zeroed by every evaluation.
+constexpr int kFinShares = 16;
+struct FinScratch {
+  double part[4 * kFinShares];
+  unsigned ticket, pad;
+};
+
+// Inputs of the integrals of one prefix field: the block partial sums of the
+// x kernel (sum v, sum w, sum f^2 per workgroup) and of the z kernel (Parseval
+// |grad|^2), and the field's k = 0 coefficient.
+struct FinArgs {
+  const double* xpart;
+  const double* zpart;
+  const double* dc;   // k = 0 coefficient of the field (its mean)
+  Scal* sc;
+  FinScratch* fs;
+  int nxb, nzb, mode, pad;
+  Geo g;
+  Phys ph;
+};
+
+// The integrals, evaluated by nb workgroups of NT threads (this is workgroup
+// b): each sums a contiguous share of the partials in a fixed order; with
+// nb > 1 the shares go to fs and the workgroup that takes the last ticket adds
+// them in workgroup order, so the result does not depend on which one
+// finishes last.  Thread 0 of the final workgroup writes V, A, E_b, E and the
+// multipliers (Calc.cu:191-263, phasefield.cpp:283-325, belastic.cpp:68-86
+// and 473-479).  red holds 4 * (NT / 64) doubles and tot 4 doubles of LDS.
+template <int NT>
+__device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, double* tot, int b, int nb) {
+  static_assert(NT % 64 == 0, "whole waves");
+  const int xs = (a.nxb + nb - 1) / nb, zs = (a.nzb + nb - 1) / nb;
+  const int x0 = b * xs, x1 = min(a.nxb, x0 + xs);
+  const int z0 = b * zs, z1 = min(a.nzb, z0 + zs);
+  // four accumulator sets per thread keep four loads in flight
+  double u[4][4] = {};
+  for (int i0 = x0 + threadIdx.x; i0 < x1; i0 += 4 * NT) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = i0 + k * NT;
+      if (i < x1) {
+        u[k][0] += a.xpart[3 * i];
+        u[k][1] += a.xpart[3 * i + 1];
+        u[k][2] += a.xpart[3 * i + 2];
+      }
+    }
+  }
+  for (int i = z0 + threadIdx.x; i < z1; i += NT) u[0][3] += a.zpart[i];
+  double s[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s[q] = (u[0][q] + u[1][q]) + (u[2][q] + u[3][q]);
+  if (nb == 1) {
+    block_sum_store<4, NT>(s, red, tot);
+    __syncthreads();
+  } else {
+    __shared__ unsigned last;
+    block_sum_store<4, NT>(s, red, a.fs->part + 4 * b);
+    __threadfence();   // publish the share before taking a ticket
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&a.fs->ticket, 1u) == unsigned(nb - 1) ? 1u : 0u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < 4) {
+      const volatile double* sh = a.fs->part;
+      double t = 0.0;
+      for (int k = 0; k < nb; ++k) t += sh[4 * k + threadIdx.x];
+      tot[threadIdx.x] = t;
+    }
+    if (threadIdx.x == 0) a.fs->ticket = 0u;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    Scal& c = *a.sc;
+    const double N = double(a.g.nr);
+    const double L3 = a.g.L3;
+    const double eps = a.ph.eps;
+    // vol: intf(d, 1, 1/2); surface: intf(s, 0, 3/(2 sqrt2)) with
+    // s = eps/2 |grad d|^2 + (d^2-1)^2/(4 eps); elastic: intf(f^2, 0, 1/(2 eps)).
+    // mean(d) is the k = 0 coefficient of the FP64 spectrum (Parseval gives
+    // mean |grad d|^2 the same way); the spatial sum is kept as sum_v.
+    const double V = (a.dc[0] + 1.0) * (0.5 * L3);
+    const double ms = 0.5 * eps * tot[3] + 0.25 / eps * (tot[1] / N);
+    const double A = ms * (3.0 / (2.0 * sqrt(2.0)) * L3);
+    const double Eb = (tot[2] / N) * (1.0 / (eps * 2) * L3);
+    double alpha = c.alpha, beta = c.beta;
+    if (a.mode == FIN_DRIFT) {
+      alpha -= (V - c.alpha0);
+      beta -= (A - c.beta0);
+      c.alpha = alpha;
+      c.beta = beta;
+    }
+    const double ve = V - alpha, se = A - beta;
+    c.V = V;
+    c.A = A;
+    c.Eb = Eb;
+    c.Eu = Eb * a.ph.eta;
+    c.E = Eb * a.ph.eta + 0.5 * a.ph.M1 * ve * ve + 0.5 * a.ph.M2 * se * se;
+    c.l1 = a.ph.M1 * ve;
+    c.l2 = a.ph.M2 * se;
+    c.sum_v = tot[0];
+    c.sum_w = tot[1];
+    c.sum_f2 = tot[2];
+    c.grad2 = tot[3];
+    c.nonfinite = !(isfinite(V) && isfinite(A) && isfinite(Eb)) ? 1 : 0;
+  }
+}
+
 }  // namespace vb

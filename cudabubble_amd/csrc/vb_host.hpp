@@ -71,7 +71,10 @@ Geo make_geo(int nx, int nz, double L);
 // concurrent streams of a batch or a request loop must not.
 hipStream_t make_queue_stream();
 
-// finalize kernel launcher (vb_solver.hip)
+// arguments of the integrals of one prefix field, and the standalone
+// finalize kernel's launcher (vb_solver.hip)
+FinArgs fin_args(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
+                 FinScratch* fs, const Geo& g, const Phys& ph, int mode);
 void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
                      const Geo& g, const Phys& ph, int mode, hipStream_t s);
 void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s);

@@ -422,14 +422,15 @@ struct ZIdx {
   bool valid;
 };
 
+// t: the workgroup's column group, blockIdx.x unless the launch has extra
+// workgroups in front
 template <int TB>
-__device__ __forceinline__ ZIdx zidx(const Geo& g) {
+__device__ __forceinline__ ZIdx zidx(const Geo& g, long long t) {
   const int c = threadIdx.x % TB;
   ZIdx r;
   if (g.aligned) {
     const int tpr = (g.nx / 2) / TB;
     const long long nnyq = (g.ny + TB - 1) / TB;
-    const long long t = blockIdx.x;
     if (t >= nnyq) {
       r.ky = int((t - nnyq) / tpr);
       r.kx = int((t - nnyq) % tpr) * TB + c;
@@ -443,7 +444,7 @@ __device__ __forceinline__ ZIdx zidx(const Geo& g) {
     }
   } else {
     const long long ncol = (long long)g.ny * g.nh;
-    long long q = (long long)blockIdx.x * TB + c;
+    long long q = t * TB + c;
     r.valid = q < ncol;
     if (!r.valid) q = ncol - 1;
     r.ky = int(q / g.nh);
@@ -451,6 +452,10 @@ __device__ __forceinline__ ZIdx zidx(const Geo& g) {
   }
   r.q = (long long)r.ky * g.pitch + r.kx;
   return r;
+}
+template <int TB>
+__device__ __forceinline__ ZIdx zidx(const Geo& g) {
+  return zidx<TB>(g, blockIdx.x);
 }
 
 // number of y / z workgroups for a tile width
@@ -509,20 +514,30 @@ struct ZlArgs {
   cpx<T>* buf;
   const cpx<T>* tw;
   Geo g;
+  FinArgs fin;   // workgroups [0, fin_on) evaluate these integrals
+  int fin_on;
 };
 
 // forward along z, multiply by -|k|^2 (laplace_kernel, Calc.cu:114-122), inverse
 template <typename T, int N, int TB = tbw<T, N>(), bool TAB = false, int V = 0>
-__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zlap(ZlArgs<T> a0, const ZlArgs<T>* tab) {
+__global__ void __launch_bounds__((TB * TFof<N, V>()), 4) k_zlap(ZlArgs<T> a0, const ZlArgs<T>* tab) {
   constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const ZlArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  if ((int)blockIdx.x < a.fin_on) {
+    // the integrals only read partials of kernels that ran before this one;
+    // their workgroups are dispatched first and overlap the column work
+    __shared__ double red[4 * ((TB * TF) / 64)];
+    __shared__ double tot[4];
+    finalize_body<TB * TF>(a.fin, red, tot, blockIdx.x, a.fin_on);
+    return;
+  }
   const Geo& g = a.g;
   const cpx<T>* __restrict__ tw = a.tw;
   cpx<T>* buf = a.buf;
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
-  const ZIdx id = zidx<TB>(g);
+  const ZIdx id = zidx<TB>(g, (long long)blockIdx.x - a.fin_on);
   const long long plane = g.pp;
   cpx<T>* col = buf + id.q;
   cpx<T> x[E];

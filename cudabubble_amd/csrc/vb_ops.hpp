@@ -65,7 +65,8 @@ struct SizeOps {
   void (*xe)(const XeArgs<T>& a, hipStream_t s);
   void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
   void (*zpass)(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
-  void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s);
+  // fin (or nullptr): integrals evaluated by one extra workgroup of the launch
+  void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s);
   void (*zprep)(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g, const cpx<T>* tw,
                 hipStream_t s);
   void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);

@@ -1,6 +1,8 @@
 // vb_size_impl.hpp -- launch wrappers for one transform length.  Included by
 // exactly one translation unit per length with VB_N defined.
 #pragma once
+#include <algorithm>
+
 #include "vb_kernels.hpp"
 #include "vb_ops.hpp"
 
@@ -65,9 +67,11 @@ struct Launch {
     else
       hipLaunchKernelGGL((k_zpass<T, N, +1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
-  static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const dim3 gr(zgrid(g)), bl(CT);
-    ZlArgs<T> a{b, tw, g};
+  static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s) {
+    // the integrals take one workgroup per 1024 x partials, up to kFinShares
+    const int nf = fin ? std::min(kFinShares, std::max(1, (fin->nxb + 1023) / 1024)) : 0;
+    const dim3 gr(zgrid(g) + unsigned(nf)), bl(CT);
+    ZlArgs<T> a{b, tw, g, fin ? *fin : FinArgs{}, nf};
     if (g_recorder) return record_launch(&launch_tab, OP_ZLAP, a, gr, bl);
     hipLaunchKernelGGL((k_zlap<T, N, TB, false, CV>), gr, bl, 0, s, a, nullptr);
   }

@@ -25,16 +25,6 @@ namespace vb {
 // finalize: block partials -> V, A, E_b, E, lambda (Calc.cu:191-263,
 // phasefield.cpp:283-325, belastic.cpp:68-86 and 473-479)
 // ---------------------------------------------------------------------------
-struct FinArgs {
-  const double* xpart;
-  const double* zpart;
-  const double* dc;   // k = 0 coefficient of the field (its mean)
-  Scal* sc;
-  int nxb, nzb, mode, pad;
-  Geo g;
-  Phys ph;
-};
-
 thread_local Recorder* g_recorder = nullptr;
 
 constexpr int kFinThreads = 1024;   // one block; wide so the partial sums load in few rounds
@@ -44,49 +34,7 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a0, const FinA
   const FinArgs a = TAB ? tab[blockIdx.z] : a0;
   __shared__ double red[4 * (kFinThreads / 64)];
   __shared__ double tot[4];
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < a.nxb; i += kFinThreads) {
-    s[0] += a.xpart[3 * i];
-    s[1] += a.xpart[3 * i + 1];
-    s[2] += a.xpart[3 * i + 2];
-  }
-  for (int i = threadIdx.x; i < a.nzb; i += kFinThreads) s[3] += a.zpart[i];
-  block_sum_store<4, kFinThreads>(s, red, tot);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    Scal& c = *a.sc;
-    const double N = double(a.g.nr);
-    const double L3 = a.g.L3;
-    const double eps = a.ph.eps;
-    // vol: intf(d, 1, 1/2); surface: intf(s, 0, 3/(2 sqrt2)) with
-    // s = eps/2 |grad d|^2 + (d^2-1)^2/(4 eps); elastic: intf(f^2, 0, 1/(2 eps)).
-    // mean(d) is the k = 0 coefficient of the FP64 spectrum (Parseval gives
-    // mean |grad d|^2 the same way); the spatial sum is kept as sum_v.
-    const double V = (a.dc[0] + 1.0) * (0.5 * L3);
-    const double ms = 0.5 * eps * tot[3] + 0.25 / eps * (tot[1] / N);
-    const double A = ms * (3.0 / (2.0 * sqrt(2.0)) * L3);
-    const double Eb = (tot[2] / N) * (1.0 / (eps * 2) * L3);
-    double alpha = c.alpha, beta = c.beta;
-    if (a.mode == FIN_DRIFT) {
-      alpha -= (V - c.alpha0);
-      beta -= (A - c.beta0);
-      c.alpha = alpha;
-      c.beta = beta;
-    }
-    const double ve = V - alpha, se = A - beta;
-    c.V = V;
-    c.A = A;
-    c.Eb = Eb;
-    c.Eu = Eb * a.ph.eta;
-    c.E = Eb * a.ph.eta + 0.5 * a.ph.M1 * ve * ve + 0.5 * a.ph.M2 * se * se;
-    c.l1 = a.ph.M1 * ve;
-    c.l2 = a.ph.M2 * se;
-    c.sum_v = tot[0];
-    c.sum_w = tot[1];
-    c.sum_f2 = tot[2];
-    c.grad2 = tot[3];
-    c.nonfinite = !(isfinite(V) && isfinite(A) && isfinite(Eb)) ? 1 : 0;
-  }
+  finalize_body<kFinThreads>(a, red, tot, 0, 1);
 }
 
 __global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0) {
@@ -103,19 +51,26 @@ static void finalize_tab(int, const void* tab, dim3 gr, dim3 bl, int lanes, hipS
   hipLaunchKernelGGL(k_finalize<true>, gr, bl, 0, s, FinArgs{}, (const FinArgs*)tab);
 }
 
-void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
-                     const Geo& g, const Phys& ph, int mode, hipStream_t s) {
+FinArgs fin_args(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
+                 FinScratch* fs, const Geo& g, const Phys& ph, int mode) {
   FinArgs a;
   a.xpart = xpart;
   a.zpart = zpart;
   a.dc = dc;
   a.sc = sc;
+  a.fs = fs;
   a.nxb = nxb;
   a.nzb = nzb;
   a.mode = mode;
   a.pad = 0;
   a.g = g;
   a.ph = ph;
+  return a;
+}
+
+void launch_finalize(const double* xpart, int nxb, const double* zpart, const double* dc, int nzb, Scal* sc,
+                     const Geo& g, const Phys& ph, int mode, hipStream_t s) {
+  const FinArgs a = fin_args(xpart, nxb, zpart, dc, nzb, sc, nullptr, g, ph, mode);
   if (g_recorder) return record_launch(&finalize_tab, OP_FIN, a, dim3(1), dim3(kFinThreads));
   hipLaunchKernelGGL(k_finalize<false>, dim3(1), dim3(kFinThreads), 0, s, a, nullptr);
 }
@@ -156,6 +111,7 @@ struct Solver final : SolverBase {
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
   DBuf<Scal> scal;
+  DBuf<FinScratch> fin_scratch;
   DBuf<cpx<T>> twx, twz;
   DBuf<cpx<double>> twxd, twzd;
   // FP64 evaluation of the integrals of the state (mixed precision only)
@@ -266,6 +222,8 @@ struct Solver final : SolverBase {
     xpart.alloc((size_t)nxb * 3);
     zpart.alloc((size_t)nzb);
     scal.alloc(1);
+    fin_scratch.alloc(1);
+    VB_HIP(hipMemsetAsync(fin_scratch.p, 0, sizeof(FinScratch), st));
     VB_HIP(hipHostMalloc((void**)&hs, sizeof(Scal), hipHostMallocDefault));
     memset(hs, 0, sizeof(Scal));
     VB_HIP(hipMemsetAsync(scal.p, 0, sizeof(Scal), st));
@@ -419,9 +377,11 @@ struct Solver final : SolverBase {
     prepped = true;
   }
 
-  void spectral_lap(cpx<T>* b) {
+  // fin: integrals to evaluate in the same launch as the z pass (one extra
+  // workgroup of it), or nullptr
+  void spectral_lap(cpx<T>* b, const FinArgs* fin) {
     L("y_fwd", [&] { ox->ypass(b, nullptr, 1, -1, g, twx.p, st); });
-    L("z_lap", [&] { oz->zlap(b, g, twz.p, st); });
+    L(fin ? "z_lap_fin" : "z_lap", [&] { oz->zlap(b, g, twz.p, fin, st); });
     L("y_inv", [&] { ox->ypass(b, nullptr, 1, +1, g, twx.p, st); });
   }
 
@@ -442,9 +402,11 @@ struct Solver final : SolverBase {
     a.inv_n = 1.0 / double(g.nr);
     a.rl = Rows{g.ny, g.pitch, g.pp};
     L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf, st); });
-    spectral_lap(Tt.p);
-    if (withf) spectral_lap(W.p);
-    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
+    // the integrals of this field (xa's and the z kernel's partials) are
+    // evaluated by an extra workgroup of the last z Laplacian
+    const FinArgs fin = fin_args(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, fin_scratch.p, g, ph, mode);
+    spectral_lap(Tt.p, withf ? nullptr : &fin);
+    if (withf) spectral_lap(W.p, &fin);
     check_launch();
     prepped = false;
     prefixed = true;
