@@ -280,6 +280,29 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "repeats": 5}))
 
 
+def copy_bandwidth(repeats=20):
+    """Read + write bytes per second of a plain 1 GiB device copy (torch's copy
+    kernel): the practical HBM ceiling of a streaming kernel on this device,
+    reported beside the 8 TB/s peak."""
+    import torch
+    n = 1024 * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(repeats):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / repeats
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / sec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,6 +372,9 @@ def main():
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
             "traffic": None, "avg_launch_us": avg * 1e6, "alg_bytes_per_launch": b}
+    copy_bw = copy_bandwidth()
+    roof["copy_GBps"] = copy_bw / 1e9
+    roof["frac_of_copy"] = achieved / copy_bw if achieved else None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%d_fp32.json" % (name, n))
     if os.path.exists(tfile):
         with open(tfile) as f:
@@ -359,7 +385,7 @@ def main():
     B_alg = 35 * C4 + 44 * R4
     t_eval = t32 / ev32
     out_extra["step_roofline"] = {"B_alg_per_evaluation": B_alg, "achieved_GBps": B_alg / t_eval / 1e9,
-                                  "frac": B_alg / t_eval / HBM_PEAK}
+                                  "frac": B_alg / t_eval / HBM_PEAK, "frac_of_copy": B_alg / t_eval / copy_bw}
     out_extra["latency_ms_per_step"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                         "n": len(lat)}
     out_extra["kernels_fp32"] = [{"name": nm, "launches": c, "ms": round(m, 3),

@@ -39,6 +39,10 @@ template <typename T, int N> constexpr int tbw() {
   return seg > occ ? seg : occ;
 }
 
+// Column tile width of the y passes and the z Laplacian, which keep one
+// column set in LDS (the stage combines keep two and stay at tbw).
+template <typename T, int N> constexpr int colw() { return tbw<T, N>(); }
+
 // Z_p = A_ext(p) + i * B_ext(p): Hermitian extension along x of two half
 // spectra.  The imaginary parts at kx = 0 and kx = N/2 are dropped, which is
 // what a real-output inverse does with them.

@@ -53,7 +53,8 @@ template <typename T>
 struct SizeOps {
   int N;
   int xlanes;      // row pairs per x workgroup
-  int ytile;       // columns per y / z workgroup
+  int ytile;       // columns per workgroup of the z stage kernels
+  int ctile;       // columns per workgroup of the y passes and the z Laplacian
   int ythreads;    // threads per y / z workgroup
   void (*xr2c_d)(const double* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, const Geo& g,
                  hipStream_t s);

@@ -15,7 +15,8 @@ struct Launch {
   static constexpr int TB = tbw<T, N>();
   static constexpr int YT = TB * TF;
   static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
-  static constexpr int CT = TB * TFof<N, CV>();       // their threads per workgroup
+  static constexpr int CB = colw<T, N>();             // their columns per workgroup
+  static constexpr int CT = CB * TFof<N, CV>();       // and threads per workgroup
   template <int SCH, int STG> static constexpr int zs_threads() { return TB * TFof<N, zsplan<T, N, SCH, STG>()>(); }
 
   static unsigned xgrid(long long npairs) { return unsigned((npairs + G - 1) / G); }
@@ -52,13 +53,13 @@ struct Launch {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
   static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const dim3 grid(unsigned(ygroups(g, TB)), unsigned(narr)), bl(CT);
+    const dim3 grid(unsigned(ygroups(g, CB)), unsigned(narr)), bl(CT);
     YArgs<T> a{b0, b1, tw, g};
     if (g_recorder) return record_launch(&launch_tab, sign < 0 ? OP_Y_FWD : OP_Y_INV, a, grid, bl);
     if (sign < 0)
-      hipLaunchKernelGGL((k_ypass<T, N, -1, TB, false, CV>), grid, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_ypass<T, N, -1, CB, false, CV>), grid, bl, 0, s, a, nullptr);
     else
-      hipLaunchKernelGGL((k_ypass<T, N, +1, TB, false, CV>), grid, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_ypass<T, N, +1, CB, false, CV>), grid, bl, 0, s, a, nullptr);
   }
   static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
@@ -70,10 +71,10 @@ struct Launch {
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s) {
     // the integrals take one workgroup per 1024 x partials, up to kFinShares
     const int nf = fin ? std::min(kFinShares, std::max(1, (fin->nxb + 1023) / 1024)) : 0;
-    const dim3 gr(zgrid(g) + unsigned(nf)), bl(CT);
+    const dim3 gr(unsigned(zgroups(g, CB)) + unsigned(nf)), bl(CT);
     ZlArgs<T> a{b, tw, g, fin ? *fin : FinArgs{}, nf};
     if (g_recorder) return record_launch(&launch_tab, OP_ZLAP, a, gr, bl);
-    hipLaunchKernelGGL((k_zlap<T, N, TB, false, CV>), gr, bl, 0, s, a, nullptr);
+    hipLaunchKernelGGL((k_zlap<T, N, CB, false, CV>), gr, bl, 0, s, a, nullptr);
   }
   static void zprep(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g,
                     const cpx<T>* tw, hipStream_t s) {
@@ -111,13 +112,13 @@ struct Launch {
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
       case OP_Y_FWD:
-        hipLaunchKernelGGL((k_ypass<T, N, -1, TB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        hipLaunchKernelGGL((k_ypass<T, N, -1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
       case OP_Y_INV:
-        hipLaunchKernelGGL((k_ypass<T, N, +1, TB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        hipLaunchKernelGGL((k_ypass<T, N, +1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
       case OP_ZLAP:
-        hipLaunchKernelGGL((k_zlap<T, N, TB, true, CV>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab);
+        hipLaunchKernelGGL((k_zlap<T, N, CB, true, CV>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab);
         break;
       case OP_ZPREP:
         hipLaunchKernelGGL((k_zprep<T, N, TB, true>), gr, bl, 0, s, ZprArgs<T>{}, (const ZprArgs<T>*)tab);
@@ -149,6 +150,7 @@ struct Launch {
     o.N = N;
     o.xlanes = G;
     o.ytile = TB;
+    o.ctile = CB;
     o.ythreads = YT;
     o.xr2c_d = &xr2c_d;
     o.xr2c = &xr2c;

@@ -192,8 +192,10 @@ struct Solver final : SolverBase {
     ozd = get_ops<double>(p.nz);
     // padded, 128-byte aligned spectrum rows when every column tile width in
     // use divides nx/2 (see Geo)
-    if ((p.nx / 2) % 16 == 0 && (p.nx / 2) % ox->ytile == 0 && (p.nx / 2) % oz->ytile == 0 &&
-        (p.nx / 2) % oxd->ytile == 0 && (p.nx / 2) % ozd->ytile == 0) {
+    bool tiles = true;
+    for (int t : {ox->ytile, oz->ytile, oxd->ytile, ozd->ytile, ox->ctile, oz->ctile, oxd->ctile, ozd->ctile})
+      tiles = tiles && (p.nx / 2) % t == 0;
+    if ((p.nx / 2) % 16 == 0 && tiles) {
       g.pitch = p.nx / 2 + 16;
       g.aligned = 1;
       // one extra aligned row unit per plane keeps the z stride away from
