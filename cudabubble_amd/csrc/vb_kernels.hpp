@@ -173,7 +173,10 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaA
   const bool valid = pair < a.npairs;
   if (!valid) pair = a.npairs - 1;
   const long long r0 = 2 * pair;
-  const double eps = a.eps, kappa = a.kappa;
+  // the pointwise terms multiply by 1/eps instead of dividing (within an ulp
+  // of the reference's quotients; an FP64 division per point is a long
+  // instruction sequence that these kernels cannot hide)
+  const double eps = a.eps, kappa = a.kappa, ieps = 1.0 / eps;
   double acc[3] = {0.0, 0.0, 0.0};
   T tm[2][E];
   T ff[WITHF ? 2 : 1][E];
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaA
         const int p = j + TF * e;
         const double v = double(x[h][e].x), lv = double(x[h][e].y);
         const double w = v * v - 1.0;
-        const double f = -lv * eps + w * v / eps;          // f_kernel, Calc.cu:160
+        const double f = -lv * eps + w * v * ieps;         // f_kernel, Calc.cu:160
         const double t1 = v * (v * v - 1.0 - kappa);       // nld_kernel1, Calc.cu:270
         if (valid) {
           acc[0] += v; acc[1] += w * w; acc[2] += f * f;
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbA
   const double l1 = a.sc->l1, l2 = a.sc->l2;
   const double eps = a.eps, ie = 1.0 / eps, ie2 = ie * ie, ie3 = ie * ie2;
   const double D = a.D, kappa = a.kappa, eta = a.eta;
-  const double c32 = 2.0 * sqrt(2.0);
+  const double c32 = 3.0 / (2.0 * sqrt(2.0));   // multiplied, not divided (see k_xa)
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     double nn[2];
@@ -274,10 +277,10 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbA
     for (int h = 0; h < 2; ++h) {
       const double dt1 = double(h ? x[e].y : x[e].x);
       const double v = double(vq[h][e].x), lv = double(vq[h][e].y);
-      const double f = -lv * eps + (v * v - 1.0) * v / eps;                          // Calc.cu:160
+      const double f = -lv * eps + (v * v - 1.0) * v * ie;                           // Calc.cu:160
       const double t1 = v * (v * v - 1.0 - kappa);
       const double nl = -dt1 * ie + D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f;   // Calc.cu:282
-      nn[h] = -l1 * 0.5 - l2 * f * 3.0 / c32 - nl * eta;                           // Calc.cu:435
+      nn[h] = -l1 * 0.5 - l2 * f * c32 - nl * eta;                                 // Calc.cu:435
     }
     x[e] = mk<T>(T(nn[0]), T(nn[1]));
   }
@@ -317,7 +320,7 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, j + TF * e);
   fft<T, N, +1>(x, j, a.tw, lds);
-  const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.t;
+  const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.t, ie2 = 1.0 / (eps * eps);
   double acc[1] = {0.0};
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -328,7 +331,7 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
       const long long r = r0 + h;
       const double lf = double(h ? x[e].y : x[e].x);
       const double v = double(a.v[r * N + p]), f = double(a.f[r * N + p]);
-      const double g0 = -lf + (3.0 * v * v - 1.0) * f / (eps * eps);    // Calc.cu:178
+      const double g0 = -lf + (3.0 * v * v - 1.0) * f * ie2;            // Calc.cu:178
       const double rhs = a.shape ? g0 : l1 * 0.5 + l2 * f + g0 * eta;    // Calc.cu:417
       nv[h] = v + t * rhs;                                               // Calc.cu:400
       acc[0] += valid ? rhs * rhs : 0.0;
