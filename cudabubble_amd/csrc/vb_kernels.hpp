@@ -7,8 +7,9 @@
 //   y-kernel  : y-inverse of both
 //   xa-kernel : x-inverse (two-for-one) -> v, lap v; f, tmp1, sums of v,
 //               (v^2-1)^2, f^2; x-forward of tmp1            (P1 fused)
-//   y, z(lap), y kernels : lap tmp1 spectrum round trip
-//   finalize  : V, A, E_b, lambda1, lambda2, E from the block partials
+//   y, z(lap), y kernels : lap tmp1 spectrum round trip; extra workgroups
+//               of the z(lap) launch evaluate V, A, E_b, lambda1, lambda2, E
+//               from the block partials (finalize_body)
 //   xb-kernel : x-inverse -> lap tmp1; nld, N; x-forward of N  (P2 fused)
 //   y-kernel  : y-forward of N
 //   zstage    : z-forward -> N^, ETD combine with the state, and the first
@@ -655,14 +656,16 @@ struct ZsArgs {
 // The N^ of a stage leaves the FFT in precision T and is stored as such, so
 // keeping it for a later stage loses nothing; the state and every combine
 // that produces it are FP64.
-// Minimum waves per SIMD for the stage combines: the FP32 ETDRK4 stages 1
-// and 3 fit four waves with a few spilled words and run faster so; stages 2
-// and 4 (FP64 state and two stage spectra live) would spill heavily.
+// Minimum waves per SIMD for the stage combines: the FP32 ETDRK4 stage 1
+// asks for four (at most a few spilled words); stage 2 fits four without
+// asking; stage 4 (FP64 state and two stage spectra live across its
+// transform) spills over 30 registers at four and runs slower than at three.
 template <typename T, int SCH, int STG> constexpr int zwaves() {
   return (sizeof(T) == 4 && SCH == 4 && STG == 1) ? 4 : 1;
 }
 // whether a stage combine loads its other operands before its forward
-// transform (overlapping them with it) or after it (fewer live registers)
+// transform (overlapping them with it) or after it (fewer live registers);
+// every stage measured faster with the loads first
 template <typename T, int SCH, int STG> constexpr bool zprefetch() { return true; }
 // plan variant of a stage combine: 16 x 16 pays for stage 3 at 256 (FP32)
 template <typename T, int N, int SCH, int STG> constexpr int zsplan() {

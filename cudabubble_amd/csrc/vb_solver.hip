@@ -3,9 +3,10 @@
 // State: the half spectrum u^ of the phase field (two buffers, current and
 // candidate, so a rejected adaptive step needs no copy-back).  Work buffers:
 // P, Q (spectra of the stage field and of its Laplacian during the inverse
-// transform), Tt (tmp1 / N pipeline), W (f pipeline, Euler only), pre and acc
-// (ETD combine accumulators), v and f (real stage field and f), block partial
-// sums, and the device scalars (V, A, E_b, E, lambda1, lambda2, alpha, beta).
+// transform), Tt (tmp1 / N pipeline), W (f pipeline, Euler only), pre, Na and
+// Nb (ETD stage spectra, group-major), v and f (real stage field and f, Euler
+// only), block partial sums, and the device scalars (V, A, E_b, E, lambda1,
+// lambda2, alpha, beta).
 //
 // A "prefix" turns the prepared stage field into v, f, lap tmp1 and the
 // integrals; a "suffix" forms N, its spectrum and the ETD combine of one
