@@ -117,7 +117,7 @@ struct Solver final : SolverBase {
   DBuf<cpx<double>> twxd, twzd;
   // FP64 evaluation of the integrals of the state (mixed precision only)
   DBuf<cpx<double>> P64, Q64, T64;
-  DBuf<double> v64, f64, xpart64, zpart64, dc64;
+  DBuf<double> xpart64, zpart64, dc64;
   DBuf<Scal> scal64;
   Scal* hs = nullptr;   // pinned host copy
   DBuf<double> rpart;   // block sums of the Euler residual
@@ -214,10 +214,14 @@ struct Solver final : SolverBase {
     for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
     for (auto* b : {&P, &Q, &Tt}) b->alloc(nc);
     for (auto* b : {&pre, &Na, &Nb}) b->alloc((size_t)zs_private(g, oz->ytile));
-    if (p.scheme == VB_EULER || p.shape_adjust) W.alloc(nc);
+    // W, v and f carry the Euler right-hand side only (run_static, shape
+    // adjust, the residual); the ETD pipeline recomputes v and f in xb
+    if (p.scheme == VB_EULER || p.shape_adjust) {
+      W.alloc(nc);
+      v.alloc(nr);
+      f.alloc(nr);
+    }
     dc.alloc(1);
-    v.alloc(nr);
-    f.alloc(nr);
     phid.alloc(nr);
     npairs = (long long)g.nz * g.ny / 2;
     nxb = int((npairs + ox->xlanes - 1) / ox->xlanes);
@@ -959,13 +963,11 @@ struct Solver final : SolverBase {
   // Integrals of the state evaluated entirely in FP64 (mixed precision):
   // FP64 z, y and x transforms of the FP64 state, FP64 f and reductions.
   void integrals64() {
-    const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
+    const size_t nc = (size_t)g.ncp;
     if (!P64.p) {
       P64.alloc(nc);
       Q64.alloc(nc);
       T64.alloc(nc);
-      v64.alloc(nr);
-      f64.alloc(nr);
       dc64.alloc(1);
       scal64.alloc(1);
     }
@@ -978,8 +980,8 @@ struct Solver final : SolverBase {
     XaArgs<double> a;
     a.P = P64.p;
     a.Q = Q64.p;
-    a.v = v64.p;
-    a.f = f64.p;
+    a.v = nullptr;   // the integrals need no real fields stored
+    a.f = nullptr;
     a.Tt = T64.p;
     a.W = nullptr;
     a.part = xpart64.p;
@@ -1063,6 +1065,8 @@ struct Solver final : SolverBase {
   double residual() override {
     if (!started) start();
     if (!W.p) W.alloc((size_t)g.ncp);
+    if (!v.p) v.alloc((size_t)g.nr);
+    if (!f.p) f.alloc((size_t)g.nr);
     if (!rpart.p) rpart.alloc((size_t)nxb);
     ensure_prefixed(true);
     XeArgs<T> e;
@@ -1107,9 +1111,10 @@ size_t solver_footprint(const vb_params& p) {
   const size_t nc = nh * (size_t)p.nx * (size_t)p.nz, nr = (size_t)p.nx * p.nx * p.nz;
   const bool mixed = p.precision == VB_FP32_MIXED;
   const size_t cs = mixed ? 8 : 16, rs = cs / 2;
-  const size_t work = 6 + ((p.scheme == VB_EULER || p.shape_adjust) ? 1 : 0);
-  size_t bytes = 3 * nc * 16 + work * nc * cs + 2 * nr * rs + nr * 8;
-  if (mixed) bytes += 3 * nc * 16 + 2 * nr * 8;   // FP64 query buffers
+  const bool euler = p.scheme == VB_EULER || p.shape_adjust;
+  const size_t work = 6 + (euler ? 1 : 0);
+  size_t bytes = 3 * nc * 16 + work * nc * cs + (euler ? 2 * nr * rs : 0) + nr * 8;
+  if (mixed) bytes += 3 * nc * 16;   // FP64 query buffers
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
   return bytes + 2 * mm * 48 + ((size_t)4 << 20);
 }
