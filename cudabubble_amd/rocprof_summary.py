@@ -3,6 +3,7 @@
     python -m cudabubble_amd.rocprof_summary stats  KERNEL_STATS.csv  --cmd "..."
     python -m cudabubble_amd.rocprof_summary traffic --fetch F.csv --write W.csv \
         --cal-fetch CF.csv --cal-write CW.csv --n 256 --cmd "..." [--json-dir DIR --json-prec fp32]
+    python -m cudabubble_amd.rocprof_summary sequence KERNEL_TRACE.csv --n 256 --prec fp32 --cmd "..."
 
 `stats` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown
 table keyed by the solver's own kernel names (the names `vb_profile` uses).
@@ -10,6 +11,10 @@ table keyed by the solver's own kernel names (the names `vb_profile` uses).
 per launch, each counter scaled by what it reads for a plain device copy of
 known size (the `bench.py --copy-calibration` run), and optionally writes one
 `traffic_<kernel>_<n>_<prec>.json` per kernel for bench.py's roofline.
+`sequence` reads a `--kernel-trace` kernel_trace.csv and reports, for the
+solver kernels of one size and precision, the average duration of each kernel
+by the kernel that ran just before it, and the idle time between consecutive
+dispatches.
 """
 import argparse
 import collections
@@ -108,6 +113,34 @@ def traffic(fetch, write, cal_fetch, cal_write, n, cmds, json_dir=None, json_pre
     return "\n".join(out) + "\n"
 
 
+def sequence(path, n, prec, cmd):
+    ev = []
+    for r in csv.DictReader(open(path)):
+        k = r["Kernel_Name"]
+        nm = solver_name(k)
+        ok = nm is not None and kernel_size(k) == str(n) and nm[1] == prec
+        ev.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), nm[0] if ok else None))
+    ev.sort()
+    by = collections.defaultdict(list)
+    gaps = []
+    for (s0, e0, k0), (s1, e1, k1) in zip(ev, ev[1:]):
+        if k0 is None or k1 is None:
+            continue
+        by[(k1, k0)].append((e1 - s1) * 1e-3)
+        gaps.append((s1 - e0) * 1e-3)
+    out = ["# Kernel durations by predecessor, rocprofv3 --kernel-trace (%d^3, %s)" % (n, prec), "",
+           "Command (repository root):", "", "    " + cmd, "",
+           "| kernel | after | launches | avg us |", "|---|---|---:|---:|"]
+    for (k1, k0), v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        out.append("| %s | %s | %d | %.1f |" % (k1, k0, len(v), sum(v) / len(v)))
+    if gaps:
+        gs = sorted(gaps)
+        out += ["", "Idle time between consecutive dispatches of these kernels: %d gaps, mean %.2f us, "
+                "median %.2f us, 99th percentile %.2f us." % (len(gs), sum(gs) / len(gs), gs[len(gs) // 2],
+                                                               gs[min(len(gs) - 1, int(0.99 * len(gs)))])]
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -121,8 +154,15 @@ def main(argv=None):
     t.add_argument("--cmd", action="append", required=True)
     t.add_argument("--json-dir")
     t.add_argument("--json-prec", help="write JSON only for this precision (fp32 / fp64)")
+    q = sub.add_parser("sequence")
+    q.add_argument("csv")
+    q.add_argument("--n", type=int, required=True)
+    q.add_argument("--prec", default="fp32")
+    q.add_argument("--cmd", required=True)
     a = ap.parse_args(argv)
-    if a.what == "stats":
+    if a.what == "sequence":
+        sys.stdout.write(sequence(a.csv, a.n, a.prec, a.cmd))
+    elif a.what == "stats":
         sys.stdout.write(stats(a.csv, a.cmd))
     else:
         sys.stdout.write(traffic(a.fetch, a.write, a.cal_fetch, a.cal_write, a.n, a.cmd, a.json_dir, a.json_prec))
