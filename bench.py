@@ -39,12 +39,15 @@ def alg_bytes(name, n, prec_bytes):
     C = modes * cs
     R = n ** 3 * prec_bytes
     table = {
-        "y_inv2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_lap_fin": 2 * C, "z_fwd": 2 * C, "z_inv": 2 * C,
-        "xa": 3 * C, "xa_f": 4 * C + 2 * R, "xb": 4 * C,
+        "y_inv2": 4 * C, "y_fwd2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_lap_fin": 2 * C,
+        "z_fwd": 2 * C, "z_inv": 2 * C,
+        # xa: P, Q in; spectra of tmp1 and s out (Euler: tmp1, f and the real v, f)
+        "xa": 4 * C, "xa_f": 4 * C + 2 * R,
         "zprep": modes * (16 + 2 * cs),
-        # stage combines: N spectrum in, stage/state spectra, next-stage P, Q out
-        "zstage1": modes * (4 * cs + 16), "zstage2": modes * (4 * cs + 16),
-        "zstage3": modes * 5 * cs, "zstage4": modes * (5 * cs + 32),
+        # stage combines: spectra of tmp1 and s in, the stage field's spectrum
+        # (state or Vh), stage/state spectra; next-stage field (Vh), P, Q out
+        "zstage1": modes * (6 * cs + 16), "zstage2": modes * (7 * cs + 16),
+        "zstage3": modes * 8 * cs, "zstage4": modes * (7 * cs + 32),
     }
     return table.get(name)
 

@@ -232,26 +232,26 @@ struct CalcImpl {
     if (prec == VB_FP32_MIXED) {
       cpx<float>* o = (cpx<float>*)s;
       oxf->xr2c((const float*)u, o, npairs, twxf.p, 1.0 / double(g.nr), g, st);
-      oxf->ypass(o, nullptr, 1, -1, g, twxf.p, st);
+      oxf->ypass(o, nullptr, 1, -1, g, twxf.p, nullptr, st);
       ozf->zpass(o, -1, g, twzf.p, st);
     } else {
       cpx<double>* o = (cpx<double>*)s;
       oxd->xr2c_d((const double*)u, o, npairs, twx.p, 1.0 / double(g.nr), g, st);
-      oxd->ypass(o, nullptr, 1, -1, g, twx.p, st);
+      oxd->ypass(o, nullptr, 1, -1, g, twx.p, nullptr, st);
       ozd->zpass(o, -1, g, twz.p, st);
     }
   }
   // backward C2R of a buffer that may be overwritten (c2r_3d_b, Calc.cu:99-104)
   void c2r_destroy(cpx<double>* s, double* u) {
     ozd->zpass(s, +1, g, twz.p, st);
-    oxd->ypass(s, nullptr, 1, +1, g, twx.p, st);
+    oxd->ypass(s, nullptr, 1, +1, g, twx.p, nullptr, st);
     oxd->xc2r_d(s, u, npairs, twx.p, g, st);
   }
   void c2r(const void* s, void* u) {
     if (prec == VB_FP32_MIXED) {
       VB_HIP(hipMemcpyAsync(fc1.p, s, sizeof(cpx<float>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));
       ozf->zpass(fc1.p, +1, g, twzf.p, st);
-      oxf->ypass(fc1.p, nullptr, 1, +1, g, twxf.p, st);
+      oxf->ypass(fc1.p, nullptr, 1, +1, g, twxf.p, nullptr, st);
       oxf->xc2r(fc1.p, (float*)u, npairs, twxf.p, g, st);
     } else {
       VB_HIP(hipMemcpyAsync(c4.p, s, sizeof(cpx<double>) * (size_t)g.nc, hipMemcpyDeviceToDevice, st));

@@ -108,18 +108,19 @@ struct FinArgs {
 // them in workgroup order, so the result does not depend on which one
 // finishes last.  Thread 0 of the final workgroup writes V, A, E_b, E and the
 // multipliers (Calc.cu:191-263, phasefield.cpp:283-325, belastic.cpp:68-86
-// and 473-479).  red holds 4 * (NT / 64) doubles and tot 4 doubles of LDS.
-template <int NT>
+// and 473-479).  red holds 4 * (NT / 64) doubles and tot 5 doubles of LDS.
+// UN accumulator sets per thread keep UN loads in flight (fewer registers in
+// kernels that carry this as a side branch).
+template <int NT, int UN = 4>
 __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, double* tot, int b, int nb) {
   static_assert(NT % 64 == 0, "whole waves");
   const int xs = (a.nxb + nb - 1) / nb, zs = (a.nzb + nb - 1) / nb;
   const int x0 = b * xs, x1 = min(a.nxb, x0 + xs);
   const int z0 = b * zs, z1 = min(a.nzb, z0 + zs);
-  // four accumulator sets per thread keep four loads in flight
-  double u[4][4] = {};
-  for (int i0 = x0 + threadIdx.x; i0 < x1; i0 += 4 * NT) {
+  double u[UN][4] = {};
+  for (int i0 = x0 + threadIdx.x; i0 < x1; i0 += UN * NT) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < UN; ++k) {
       const int i = i0 + k * NT;
       if (i < x1) {
         u[k][0] += a.xpart[3 * i];
@@ -131,18 +132,22 @@ __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, dou
   for (int i = z0 + threadIdx.x; i < z1; i += NT) u[0][3] += a.zpart[i];
   double s[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) s[q] = (u[0][q] + u[1][q]) + (u[2][q] + u[3][q]);
+  for (int q = 0; q < 4; ++q) {
+    s[q] = u[0][q];
+#pragma unroll
+    for (int k = 1; k < UN; ++k) s[q] += u[k][q];
+  }
   if (nb == 1) {
     block_sum_store<4, NT>(s, red, tot);
     __syncthreads();
   } else {
-    __shared__ unsigned last;
+    unsigned* last = reinterpret_cast<unsigned*>(tot + 4);
     block_sum_store<4, NT>(s, red, a.fs->part + 4 * b);
     __threadfence();   // publish the share before taking a ticket
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&a.fs->ticket, 1u) == unsigned(nb - 1) ? 1u : 0u;
+    if (threadIdx.x == 0) *last = atomicAdd(&a.fs->ticket, 1u) == unsigned(nb - 1) ? 1u : 0u;
     __syncthreads();
-    if (!last) return;
+    if (!*last) return;
     __threadfence();
     if (threadIdx.x < 4) {
       const volatile double* sh = a.fs->part;

@@ -26,7 +26,7 @@ template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
 // Minimum waves per SIMD requested for the fused x kernels: in FP32 four
 // (the register budget that allows it costs at most a few spilled words and
 // gains more in latency hiding than it loses); FP64 keeps the full budget.
-template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
+template <typename T, bool WITHS = false> constexpr int xwaves() { return (sizeof(T) == 4 && !WITHS) ? 4 : 1; }
 
 // Plan variant of the FP32 column kernels (y passes, z Laplacian, stage-3
 // combine) at 256: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
@@ -151,18 +151,26 @@ struct XaArgs {
   T* f;
   cpx<T>* Tt;        // out: x-forward of tmp1, times inv_n
   cpx<T>* W;         // out (Euler only): x-forward of f, times inv_n
+  cpx<T>* Ts;        // out (WITHS): x-forward of s, times inv_n
   double* part;      // [gridDim.x][3]
   const cpx<T>* tw;
   long long npairs;
-  double eps, kappa, inv_n;
+  double eps, kappa, D, inv_n;
   Rows rl;
 };
 
 // Every pipeline kernel takes its argument struct by value, or (TAB, a
 // batched launch over the solves of a size group) from a per-solve table in
 // device memory indexed by blockIdx.z.
-template <typename T, int N, bool WITHF, bool TAB = false>
-__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
+// WITHS: the spectral form of the nonlinear term.  nld = -lap(tmp1)/eps + s
+// with s = D tmp1/eps^3 + (3v^2 - D - 1) f/eps^2 (Calc.cu:278-282), and the
+// forward transform is linear, so N^ = -lambda1/2 delta_k0 - lambda2 c F(f)
+// - eta F(s) - (eta/eps) |k|^2 F(tmp1), with F(f) = eps |k|^2 v^ +
+// (F(tmp1) + kappa v^)/eps.  This kernel writes the x-forward of tmp1 and s;
+// the stage combine forms N^ from their spectra, so lap(tmp1) never goes back
+// to real space.
+template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false>
+__global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
@@ -178,9 +186,11 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaA
   // of the reference's quotients; an FP64 division per point is a long
   // instruction sequence that these kernels cannot hide)
   const double eps = a.eps, kappa = a.kappa, ieps = 1.0 / eps;
+  const double D = a.D, ie2 = ieps * ieps, ie3 = ie2 * ieps;
   double acc[3] = {0.0, 0.0, 0.0};
   T tm[2][E];
   T ff[WITHF ? 2 : 1][E];
+  T ss[WITHS ? 2 : 1][E];
   {
     cpx<T> x[2][E];
 #pragma unroll
@@ -209,10 +219,24 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xa(XaArgs<T> a0, const XaA
         }
         tm[h][e] = T(t1);
         if constexpr (WITHF) ff[h][e] = T(f);
+        if constexpr (WITHS) ss[h][e] = T(D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f);   // Calc.cu:282
       }
     }
   }
-  {
+  if constexpr (WITHS) {   // one transform after the other: fewer live registers
+    {
+      cpx<T> z[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
+      fft<T, N, -1>(z, j, a.tw, lds);
+      unpack_store<T, N>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    }
+    cpx<T> z[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) z[e] = mk<T>(ss[0][e], ss[1][e]);
+    fft<T, N, -1>(z, j, a.tw, lds2);
+    unpack_store<T, N>(z, j, lds2, a.Ts + a.rl.off(r0), a.Ts + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+  } else {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
@@ -358,8 +382,10 @@ struct YIdx {
   bool valid;
 };
 
+// t: the workgroup's column group (blockIdx.x unless the launch has extra
+// workgroups in front)
 template <int TB>
-__device__ __forceinline__ YIdx yidx(const Geo& g) {
+__device__ __forceinline__ YIdx yidx(const Geo& g, long long t) {
   const int c = threadIdx.x % TB;
   YIdx r;
   int z, kx;
@@ -368,7 +394,6 @@ __device__ __forceinline__ YIdx yidx(const Geo& g) {
     // with the main groups instead of trailing them
     const int tpr = (g.nx / 2) / TB;
     const long long nnyq = (g.nz + TB - 1) / TB;
-    const long long t = blockIdx.x;
     if (t >= nnyq) {
       z = int((t - nnyq) / tpr);
       kx = int((t - nnyq) % tpr) * TB + c;
@@ -381,7 +406,7 @@ __device__ __forceinline__ YIdx yidx(const Geo& g) {
     }
   } else {
     const long long batch = (long long)g.nz * g.nh;
-    long long b = (long long)blockIdx.x * TB + c;
+    long long b = t * TB + c;
     r.valid = b < batch;
     if (!r.valid) b = batch - 1;
     z = int(b / g.nh);
@@ -390,6 +415,10 @@ __device__ __forceinline__ YIdx yidx(const Geo& g) {
   r.base = (long long)z * g.pp + kx;
   return r;
 }
+template <int TB>
+__device__ __forceinline__ YIdx yidx(const Geo& g) {
+  return yidx<TB>(g, blockIdx.x);
+}
 
 template <typename T>
 struct YArgs {
@@ -397,6 +426,8 @@ struct YArgs {
   cpx<T>* b1;   // blockIdx.y = 1 transforms this one
   const cpx<T>* tw;
   Geo g;
+  FinArgs fin;  // workgroups [0, fin_on) of blockIdx.y = 0 evaluate these integrals
+  int fin_on;
 };
 
 template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false, int V = 0>
@@ -404,12 +435,21 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
   constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
   const YArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  if ((int)blockIdx.x < a.fin_on) {
+    // the integrals read only partials of kernels that ran before this one;
+    // their workgroups are dispatched first and overlap the column work
+    if (blockIdx.y == 0) {
+      double* red = reinterpret_cast<double*>(sm);   // the column LDS, unused here
+      finalize_body<TB * TF, 2>(a.fin, red, red + 4 * ((TB * TF) / 64), blockIdx.x, a.fin_on);
+    }
+    return;
+  }
   const Geo& g = a.g;
   const cpx<T>* __restrict__ tw = a.tw;
   cpx<T>* buf = blockIdx.y ? a.b1 : a.b0;
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
-  const YIdx id = yidx<TB>(g);
+  const YIdx id = yidx<TB>(g, (long long)blockIdx.x - a.fin_on);
   cpx<T>* col = buf + id.base;
   cpx<T> x[E];
 #pragma unroll
@@ -536,7 +576,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), 4) k_zlap(ZlArgs<T> a0, c
     // the integrals only read partials of kernels that ran before this one;
     // their workgroups are dispatched first and overlap the column work
     __shared__ double red[4 * ((TB * TF) / 64)];
-    __shared__ double tot[4];
+    __shared__ double tot[5];
     finalize_body<TB * TF>(a.fin, red, tot, blockIdx.x, a.fin_on);
     return;
   }
@@ -635,7 +675,12 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
 
 template <typename T>
 struct ZsArgs {
-  const cpx<T>* U;        // x,y-forward of N (normalised)
+  const cpx<T>* U;        // x,y-forward of N (normalised); NF: of tmp1
+  const cpx<T>* U2;       // NF: x,y-forward of s
+  cpx<T>* Vh;             // NF: spectrum of the stage field (stages after the
+                          // first) and of the next one, group-major, in place
+  const Scal* sc;         // NF: lambda1, lambda2 of the stage field
+  double eps, kappa, eta; // NF
   const cpx<double>* uh;  // state spectrum u^
   cpx<T>* pre;            // ETDRK4: e1 a^ - q N^u (stage 1 -> 3)
   cpx<T>* Na;             // ETDRK4: N^a (stage 2 -> 4); ETDRK2: N^u (stage 1 -> 2)
@@ -660,21 +705,25 @@ struct ZsArgs {
 // asks for four (at most a few spilled words); stage 2 fits four without
 // asking; stage 4 (FP64 state and two stage spectra live across its
 // transform) spills over 30 registers at four and runs slower than at three.
-template <typename T, int SCH, int STG> constexpr int zwaves() {
-  return (sizeof(T) == 4 && SCH == 4 && STG == 1) ? 4 : 1;
+template <typename T, int SCH, int STG, bool NF = false> constexpr int zwaves() {
+  return (sizeof(T) == 4 && SCH == 4 && STG == 1 && !NF) ? 4 : 1;
 }
 // whether a stage combine loads its other operands before its forward
 // transform (overlapping them with it) or after it (fewer live registers);
 // every stage measured faster with the loads first
-template <typename T, int SCH, int STG> constexpr bool zprefetch() { return true; }
+template <typename T, int SCH, int STG, bool NF = false> constexpr bool zprefetch() { return !NF; }
 // plan variant of a stage combine: 16 x 16 pays for stage 3 at 256 (FP32)
-template <typename T, int N, int SCH, int STG> constexpr int zsplan() {
-  return (SCH == 4 && STG == 3) ? colplan<T, N>() : 0;
+template <typename T, int N, int SCH, int STG, bool NF = false> constexpr int zsplan() {
+  return (SCH == 4 && STG == 3 && !NF) ? colplan<T, N>() : 0;
 }
 
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false,
-          int V = zsplan<T, N, SCH, STG>()>
-__global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) k_zstage(ZsArgs<T> a0,
+// NF: the stage's N^ is formed here from the spectra of tmp1 and s (see
+// k_xa), the stage field's spectrum v^ (the state u^ in the first stage, Vh
+// after it) and lambda1, lambda2; the stage also stores the next stage
+// field's spectrum in Vh.
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, bool NF = false,
+          int V = zsplan<T, N, SCH, STG, NF>()>
+__global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG, NF>())) k_zstage(ZsArgs<T> a0,
                                                                                      const ZsArgs<T>* tab) {
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, V>::E, TF = N / E;
@@ -691,13 +740,18 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
   // contiguous run (see zs_private), which keeps the strided z traffic of a
   // stage to the spectra that other kernels share
   const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
-  cpx<T> x[E];
+  cpx<T> x[E], x2[NF ? E : 1];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
+  for (int e = 0; e < E; ++e) {
+    x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
+    if constexpr (NF) x2[e] = a.U2[id.q + (long long)(j + TF * e) * plane];
+  }
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
+  constexpr bool READS_V = NF && STG > 1;   // stage field spectrum from Vh
   cpx<double> ur[USES_U ? E : 1];
   cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? E : 1];
   cpx<T> pb[(SCH == 4 && STG == 4) ? E : 1];
+  cpx<T> pv[READS_V ? E : 1];
   auto load_operands = [&]() {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -707,11 +761,36 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
       if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[ib];
       if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[ib]; pb[e] = a.Nb[ib]; }
       if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[ib];
+      if constexpr (READS_V) pv[e] = a.Vh[ib];
     }
   };
-  if constexpr (zprefetch<T, SCH, STG>()) load_operands();
-  fft<T, N, -1, V>(x, j, a.tw, lds);   // x = N^ of this stage
-  if constexpr (!zprefetch<T, SCH, STG>()) load_operands();
+  if constexpr (zprefetch<T, SCH, STG, NF>()) load_operands();
+  double l2c = 0.0, ie = 0.0;
+  if constexpr (NF) {
+    // Calc.cu:428-444 with nld (Calc.cu:265-297) moved to spectral space:
+    // N^ = -lambda1/2 delta_k0 - lambda2 3/(2 sqrt 2) F(f) - eta F(s)
+    //      - (eta/eps) |k|^2 F(tmp1),  F(f) = eps |k|^2 v^ + (F(tmp1) + kappa v^)/eps.
+    // Everything but the v^ terms is formed right after the transforms, so
+    // the spectrum of s is dead before the combine's operands load:
+    // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
+    fft2<T, N, -1, V>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
+    const double l1h = 0.5 * a.sc->l1;
+    l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
+    ie = 1.0 / a.eps;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int kz = j + TF * e;
+      const double k2 = double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
+      const double at = -(l2c + a.eta * k2) * ie;   // on F(tmp1)
+      double pr = at * double(x[e].x) - a.eta * double(x2[e].x);
+      const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
+      if (id.q == 0 && kz == 0) pr -= l1h;
+      x[e] = mk<T>(T(pr), T(pi));
+    }
+  } else {
+    fft<T, N, -1, V>(x, j, a.tw, lds);              // N^ of this stage
+  }
+  if constexpr (!zprefetch<T, SCH, STG, NF>()) load_operands();
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll
@@ -720,7 +799,25 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
     const long long i = id.q + (long long)kz * plane;
     const long long ib = ib0 + (long long)kz * TB;
     const int m = mindex(g, id.kx, id.ky, kz);
-    const double nr = double(x[e].x), ni = double(x[e].y);
+    double nr, ni;   // N^ of this stage
+    if constexpr (NF) {
+      const double k2 = double(m) * g.kfac2;
+      const double av = -l2c * (a.eps * k2 + a.kappa * ie);
+      double vr, vi;
+      if constexpr (STG == 1) {
+        vr = ur[e].x;
+        vi = ur[e].y;
+      } else {
+        vr = double(pv[e].x);
+        vi = double(pv[e].y);
+      }
+      nr = double(x[e].x) + av * vr;
+      ni = double(x[e].y) + av * vi;
+      x[e] = mk<T>(T(nr), T(ni));
+    } else {
+      nr = double(x[e].x);
+      ni = double(x[e].y);
+    }
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
       const double2 c = a.c1[m];                      // e1, q
@@ -764,6 +861,9 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG>())) 
       fr = ar + c2 * (nr - double(nu.x));
       fi = ai + c2 * (ni - double(nu.y));
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
+    }
+    if constexpr (NF && STG < SCH) {
+      if (id.valid) a.Vh[ib] = mk<T>(T(fr), T(fi));
     }
     if (e == 0) dc0 = fr;
     next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);

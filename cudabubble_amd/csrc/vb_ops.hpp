@@ -17,9 +17,12 @@ namespace vb {
 // then issues one launch per recorded position for all solves of a size
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
-  OP_XA, OP_XA_F, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
-  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN
+  OP_XA, OP_XA_F, OP_XA_S, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22,
+  OP_ZS41N, OP_ZS42N, OP_ZS43N, OP_ZS44N, OP_ZS21N, OP_ZS22N, OP_FIN
 };
+// x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N)
+enum XaMode { XA_PLAIN = 0, XA_WITHF = 1, XA_WITHS = 2 };
 struct RecLaunch {
   // issues the table form of the recorded kernel over `lanes` solves
   void (*launch_tab)(int op, const void* tab, dim3 grid, dim3 block, int lanes, hipStream_t s);
@@ -61,16 +64,19 @@ struct SizeOps {
   void (*xr2c)(const T* in, cpx<T>* out, long long npairs, const cpx<T>* tw, double scl, const Geo& g, hipStream_t s);
   void (*xc2r_d)(const cpx<T>* in, double* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
   void (*xc2r)(const cpx<T>* in, T* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
-  void (*xa)(const XaArgs<T>& a, bool withf, hipStream_t s);
+  void (*xa)(const XaArgs<T>& a, int mode, hipStream_t s);   // mode: XaMode
   void (*xb)(const XbArgs<T>& a, hipStream_t s);
   void (*xe)(const XeArgs<T>& a, hipStream_t s);
-  void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
+  // fin (or nullptr): integrals evaluated by extra workgroups of the launch
+  void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, const FinArgs* fin,
+                hipStream_t s);
   void (*zpass)(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
   // fin (or nullptr): integrals evaluated by one extra workgroup of the launch
   void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s);
   void (*zprep)(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g, const cpx<T>* tw,
                 hipStream_t s);
-  void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);
+  // nf: N^ formed in the combine from the spectra of tmp1 and s (k_zstage)
+  void (*zstage)(const ZsArgs<T>& a, int sch, int stg, bool nf, hipStream_t s);
 };
 
 template <typename T, int N> const SizeOps<T>* ops_for();

@@ -17,7 +17,13 @@ struct Launch {
   static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
   static constexpr int CB = colw<T, N>();             // their columns per workgroup
   static constexpr int CT = CB * TFof<N, CV>();       // and threads per workgroup
-  template <int SCH, int STG> static constexpr int zs_threads() { return TB * TFof<N, zsplan<T, N, SCH, STG>()>(); }
+  template <int SCH, int STG, bool NF = false> static constexpr int zs_threads() {
+    return TB * TFof<N, zsplan<T, N, SCH, STG, NF>()>();
+  }
+  // workgroups of the integrals in a launch that evaluates them (nullptr: none)
+  static int fin_groups(const FinArgs* fin) {
+    return fin ? std::min(kFinShares, std::max(1, (fin->nxb + 1023) / 1024)) : 0;
+  }
 
   static unsigned xgrid(long long npairs) { return unsigned((npairs + G - 1) / G); }
 
@@ -36,11 +42,14 @@ struct Launch {
   static void xc2r(const cpx<T>* in, T* out, long long np, const cpx<T>* tw, const Geo& g, hipStream_t s) {
     hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
-  static void xa(const XaArgs<T>& a, bool withf, hipStream_t s) {
+  static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
     const dim3 gr(xgrid(a.npairs)), bl(XWG);
-    if (g_recorder) return record_launch(&launch_tab, withf ? OP_XA_F : OP_XA, a, gr, bl);
-    if (withf)
+    if (g_recorder)
+      return record_launch(&launch_tab, mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : OP_XA, a, gr, bl);
+    if (mode == XA_WITHF)
       hipLaunchKernelGGL((k_xa<T, N, true>), gr, bl, 0, s, a, nullptr);
+    else if (mode == XA_WITHS)
+      hipLaunchKernelGGL((k_xa<T, N, false, false, true>), gr, bl, 0, s, a, nullptr);
     else
       hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
   }
@@ -52,9 +61,11 @@ struct Launch {
   static void xe(const XeArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
-  static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
-    const dim3 grid(unsigned(ygroups(g, CB)), unsigned(narr)), bl(CT);
-    YArgs<T> a{b0, b1, tw, g};
+  static void ypass(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, const FinArgs* fin,
+                    hipStream_t s) {
+    const int nf = fin_groups(fin);
+    const dim3 grid(unsigned(ygroups(g, CB)) + unsigned(nf), unsigned(narr)), bl(CT);
+    YArgs<T> a{b0, b1, tw, g, fin ? *fin : FinArgs{}, nf};
     if (g_recorder) return record_launch(&launch_tab, sign < 0 ? OP_Y_FWD : OP_Y_INV, a, grid, bl);
     if (sign < 0)
       hipLaunchKernelGGL((k_ypass<T, N, -1, CB, false, CV>), grid, bl, 0, s, a, nullptr);
@@ -69,8 +80,7 @@ struct Launch {
       hipLaunchKernelGGL((k_zpass<T, N, +1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s) {
-    // the integrals take one workgroup per 1024 x partials, up to kFinShares
-    const int nf = fin ? std::min(kFinShares, std::max(1, (fin->nxb + 1023) / 1024)) : 0;
+    const int nf = fin_groups(fin);
     const dim3 gr(unsigned(zgroups(g, CB)) + unsigned(nf)), bl(CT);
     ZlArgs<T> a{b, tw, g, fin ? *fin : FinArgs{}, nf};
     if (g_recorder) return record_launch(&launch_tab, OP_ZLAP, a, gr, bl);
@@ -83,25 +93,36 @@ struct Launch {
     if (g_recorder) return record_launch(&launch_tab, OP_ZPREP, a, gr, bl);
     hipLaunchKernelGGL((k_zprep<T, N>), gr, bl, 0, s, a, nullptr);
   }
-  static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
-    const dim3 gr(zgrid(a.g));
-    int nt = YT;
-    if (sch == 4) nt = stg == 1 ? zs_threads<4, 1>() : stg == 2 ? zs_threads<4, 2>() : stg == 3 ? zs_threads<4, 3>()
-                                                                                           : zs_threads<4, 4>();
-    else nt = stg == 1 ? zs_threads<2, 1>() : zs_threads<2, 2>();
-    const dim3 bl = dim3((unsigned)nt);
-    if (g_recorder) return record_launch(&launch_tab, sch == 4 ? OP_ZS41 + (stg - 1) : OP_ZS21 + (stg - 1), a, gr, bl);
+  template <int SCH, int STG, bool TAB, bool NF>
+  static void zs_launch(dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
+    const dim3 bl((unsigned)zs_threads<SCH, STG, NF>());
+    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB, NF>), gr, bl, 0, s, a, tab);
+  }
+  template <bool TAB, bool NF>
+  static void zs_dispatch(int sch, int stg, dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
     if (sch == 4) {
       switch (stg) {
-        case 1: hipLaunchKernelGGL((k_zstage<T, N, 4, 1>), gr, bl, 0, s, a, nullptr); break;
-        case 2: hipLaunchKernelGGL((k_zstage<T, N, 4, 2>), gr, bl, 0, s, a, nullptr); break;
-        case 3: hipLaunchKernelGGL((k_zstage<T, N, 4, 3>), gr, bl, 0, s, a, nullptr); break;
-        default: hipLaunchKernelGGL((k_zstage<T, N, 4, 4>), gr, bl, 0, s, a, nullptr); break;
+        case 1: zs_launch<4, 1, TAB, NF>(gr, a, tab, s); break;
+        case 2: zs_launch<4, 2, TAB, NF>(gr, a, tab, s); break;
+        case 3: zs_launch<4, 3, TAB, NF>(gr, a, tab, s); break;
+        default: zs_launch<4, 4, TAB, NF>(gr, a, tab, s); break;
       }
+    } else if (stg == 1) {
+      zs_launch<2, 1, TAB, NF>(gr, a, tab, s);
     } else {
-      if (stg == 1) hipLaunchKernelGGL((k_zstage<T, N, 2, 1>), gr, bl, 0, s, a, nullptr);
-      else hipLaunchKernelGGL((k_zstage<T, N, 2, 2>), gr, bl, 0, s, a, nullptr);
+      zs_launch<2, 2, TAB, NF>(gr, a, tab, s);
     }
+  }
+  static void zstage(const ZsArgs<T>& a, int sch, int stg, bool nf, hipStream_t s) {
+    const dim3 gr(zgrid(a.g));
+    if (g_recorder) {
+      const int op = (nf ? (sch == 4 ? OP_ZS41N : OP_ZS21N) : (sch == 4 ? OP_ZS41 : OP_ZS21)) + (stg - 1);
+      return record_launch(&launch_tab, op, a, gr, dim3(1));   // the table launch picks the block
+    }
+    if (nf)
+      zs_dispatch<false, true>(sch, stg, gr, a, nullptr, s);
+    else
+      zs_dispatch<false, false>(sch, stg, gr, a, nullptr, s);
   }
 
   // the table form of a recorded launch, over `lanes` solves (blockIdx.z)
@@ -110,6 +131,9 @@ struct Launch {
     switch (op) {
       case OP_XA: hipLaunchKernelGGL((k_xa<T, N, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
+      case OP_XA_S:
+        hipLaunchKernelGGL((k_xa<T, N, false, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        break;
       case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
       case OP_Y_FWD:
         hipLaunchKernelGGL((k_ypass<T, N, -1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
@@ -123,23 +147,17 @@ struct Launch {
       case OP_ZPREP:
         hipLaunchKernelGGL((k_zprep<T, N, TB, true>), gr, bl, 0, s, ZprArgs<T>{}, (const ZprArgs<T>*)tab);
         break;
-      case OP_ZS41:
-        hipLaunchKernelGGL((k_zstage<T, N, 4, 1, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+      case OP_ZS41: case OP_ZS42: case OP_ZS43: case OP_ZS44:
+        zs_dispatch<true, false>(4, op - OP_ZS41 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
-      case OP_ZS42:
-        hipLaunchKernelGGL((k_zstage<T, N, 4, 2, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+      case OP_ZS21: case OP_ZS22:
+        zs_dispatch<true, false>(2, op - OP_ZS21 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
-      case OP_ZS43:
-        hipLaunchKernelGGL((k_zstage<T, N, 4, 3, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+      case OP_ZS41N: case OP_ZS42N: case OP_ZS43N: case OP_ZS44N:
+        zs_dispatch<true, true>(4, op - OP_ZS41N + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
-      case OP_ZS44:
-        hipLaunchKernelGGL((k_zstage<T, N, 4, 4, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
-        break;
-      case OP_ZS21:
-        hipLaunchKernelGGL((k_zstage<T, N, 2, 1, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
-        break;
-      case OP_ZS22:
-        hipLaunchKernelGGL((k_zstage<T, N, 2, 2, TB, true>), gr, bl, 0, s, ZsArgs<T>{}, (const ZsArgs<T>*)tab);
+      case OP_ZS21N: case OP_ZS22N:
+        zs_dispatch<true, true>(2, op - OP_ZS21N + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
       default: break;
     }

@@ -34,7 +34,7 @@ template <bool TAB>
 __global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a0, const FinArgs* tab) {
   const FinArgs a = TAB ? tab[blockIdx.z] : a0;
   __shared__ double red[4 * (kFinThreads / 64)];
-  __shared__ double tot[4];
+  __shared__ double tot[5];
   finalize_body<kFinThreads>(a, red, tot, 0, 1);
 }
 
@@ -107,8 +107,8 @@ struct Solver final : SolverBase {
   const SizeOps<double>* oxd = nullptr;
   const SizeOps<double>* ozd = nullptr;
   DBuf<cpx<double>> uh[2], S;            // state (FP64) and a transform scratch
-  DBuf<cpx<T>> P, Q, Tt, W;              // transform work buffers
-  DBuf<cpx<T>> pre, Na, Nb;              // stage spectra kept for later stages
+  DBuf<cpx<T>> P, Q, Tt, Ts, W;          // transform work buffers
+  DBuf<cpx<T>> pre, Na, Nb, Vh;          // stage spectra kept for later stages (group-major)
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
   DBuf<Scal> scal;
@@ -133,8 +133,8 @@ struct Solver final : SolverBase {
   // pipeline state
   int cur = 0;
   bool prepped = false;     // P, Q hold zprep of uh[cur]
-  bool prefixed = false;    // prefix of uh[cur] done (v, f, Tt, scal valid)
-  bool prefixed_f = false;  // ... including W (Euler)
+  bool prefixed = false;    // ETD prefix of uh[cur] done (scal; Tt, Ts: x,y-forward of tmp1, s)
+  bool prefixed_f = false;  // Euler prefix of uh[cur] done (scal; v, f; W: x-spectra of lap f)
   // driver state
   bool started = false;
   double time_step = 0, time_current = 0;
@@ -212,8 +212,8 @@ struct Solver final : SolverBase {
     }
     const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
     for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
-    for (auto* b : {&P, &Q, &Tt}) b->alloc(nc);
-    for (auto* b : {&pre, &Na, &Nb}) b->alloc((size_t)zs_private(g, oz->ytile));
+    for (auto* b : {&P, &Q, &Tt, &Ts}) b->alloc(nc);
+    for (auto* b : {&pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
     // W, v and f carry the Euler right-hand side only (run_static, shape
     // adjust, the residual); the ETD pipeline recomputes v and f in xb
     if (p.scheme == VB_EULER || p.shape_adjust) {
@@ -384,16 +384,21 @@ struct Solver final : SolverBase {
     prepped = true;
   }
 
-  // fin: integrals to evaluate in the same launch as the z pass (one extra
-  // workgroup of it), or nullptr
+  // fin: integrals to evaluate in the same launch as the z pass (extra
+  // workgroups of it), or nullptr
   void spectral_lap(cpx<T>* b, const FinArgs* fin) {
-    L("y_fwd", [&] { ox->ypass(b, nullptr, 1, -1, g, twx.p, st); });
+    L("y_fwd", [&] { ox->ypass(b, nullptr, 1, -1, g, twx.p, nullptr, st); });
     L(fin ? "z_lap_fin" : "z_lap", [&] { oz->zlap(b, g, twz.p, fin, st); });
-    L("y_inv", [&] { ox->ypass(b, nullptr, 1, +1, g, twx.p, st); });
+    L("y_inv", [&] { ox->ypass(b, nullptr, 1, +1, g, twx.p, nullptr, st); });
   }
 
+  // The prefix of the prepared stage field (P, Q: its z-inverse spectra and
+  // those of its Laplacian): the integrals in scal, and
+  //  - ETD (withf = false): Tt, Ts = x,y-forward of tmp1 and s, from which the
+  //    stage combine forms N^ (k_xa, k_zstage);
+  //  - Euler (withf = true): v, f and W = x-spectra of lap f (k_xe).
   void prefix(int mode, bool withf) {
-    L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, st); });
+    L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, nullptr, st); });
     XaArgs<T> a;
     a.P = P.p;
     a.Q = Q.p;
@@ -401,22 +406,27 @@ struct Solver final : SolverBase {
     a.f = f.p;
     a.Tt = Tt.p;
     a.W = W.p;
+    a.Ts = Ts.p;
     a.part = xpart.p;
     a.tw = twx.p;
     a.npairs = npairs;
     a.eps = p.eps;
     a.kappa = p.kappa;
+    a.D = p.D;
     a.inv_n = 1.0 / double(g.nr);
     a.rl = Rows{g.ny, g.pitch, g.pp};
-    L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf, st); });
+    L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf ? XA_WITHF : XA_WITHS, st); });
     // the integrals of this field (xa's and the z kernel's partials) are
-    // evaluated by an extra workgroup of the last z Laplacian
+    // evaluated by extra workgroups of the next column pass
     const FinArgs fin = fin_args(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, fin_scratch.p, g, ph, mode);
-    spectral_lap(Tt.p, withf ? nullptr : &fin);
-    if (withf) spectral_lap(W.p, &fin);
+    if (withf) {
+      spectral_lap(W.p, &fin);
+    } else {
+      L("y_fwd2", [&] { ox->ypass(Tt.p, Ts.p, 2, -1, g, twx.p, &fin, st); });
+    }
     check_launch();
     prepped = false;
-    prefixed = true;
+    prefixed = !withf;
     prefixed_f = withf;
   }
 
@@ -425,24 +435,17 @@ struct Solver final : SolverBase {
     check_launch();
   }
 
+  // The combine of stage stg: z-forward of tmp1 and s, N^, the ETD combine,
+  // and the z-inverse of the next stage field into P, Q.
   void suffix(int sch, int stg, const Lut* lt, int out) {
-    XbArgs<T> b;
-    b.Tt = Tt.p;
-    b.P = P.p;
-    b.Q = Q.p;
-    b.sc = scal.p;
-    b.tw = twx.p;
-    b.npairs = npairs;
-    b.eps = p.eps;
-    b.kappa = p.kappa;
-    b.D = p.D;
-    b.eta = p.eta;
-    b.inv_n = 1.0 / double(g.nr);
-    b.rl = Rows{g.ny, g.pitch, g.pp};
-    L("xb", [&] { ox->xb(b, st); });
-    L("y_fwd", [&] { ox->ypass(Tt.p, nullptr, 1, -1, g, twx.p, st); });
     ZsArgs<T> z;
     z.U = Tt.p;
+    z.U2 = Ts.p;
+    z.Vh = Vh.p;
+    z.sc = scal.p;
+    z.eps = p.eps;
+    z.kappa = p.kappa;
+    z.eta = p.eta;
     z.uh = uh[cur].p;
     z.pre = pre.p;
     z.Na = Na.p;
@@ -457,7 +460,7 @@ struct Solver final : SolverBase {
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
-    L(names[stg], [&] { oz->zstage(z, sch, stg, st); });
+    L(names[stg], [&] { oz->zstage(z, sch, stg, true, st); });
     check_launch();
     prefixed = false;
     prefixed_f = false;
@@ -465,9 +468,14 @@ struct Solver final : SolverBase {
   }
 
   void ensure_prefixed(bool withf = false) {
-    if (prefixed && (!withf || prefixed_f)) return;
+    if (withf ? prefixed_f : prefixed) return;
     if (!prepped) prep(uh[cur].p);
     prefix(FIN_PLAIN, withf);
+  }
+  // the integrals of uh[cur] in scal: either prefix provides them
+  void ensure_integrals() {
+    if (prefixed || prefixed_f) return;
+    ensure_prefixed(p.scheme == VB_EULER);
   }
 
   // one ETDRK4 / ETDRK2 evaluation from uh[cur] into uh[1-cur]; on return P, Q
@@ -505,7 +513,7 @@ struct Solver final : SolverBase {
     e.rl = Rows{g.ny, g.pitch, g.pp};
     e.part = nullptr;
     L("xe", [&] { ox->xe(e, st); });
-    L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, st); });
+    L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, nullptr, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
     L("widen", [&] {
       hipLaunchKernelGGL(k_widen<T>, dim3(unsigned((g.ncp + 255) / 256)), dim3(256), 0, st, W.p, uh[out].p, g.ncp);
@@ -793,7 +801,7 @@ struct Solver final : SolverBase {
   void finish_steps(vb_step_report* rep) override {
     if (drift_mode()) sync_targets_from_device();
     if (rep->status == VB_OK) {
-      ensure_prefixed();
+      ensure_prefixed();   // the next step's first stage needs it anyway
       read_scal();
       if (hs->nonfinite) rep->status = VB_ERR_NONFINITE;
     }
@@ -976,7 +984,7 @@ struct Solver final : SolverBase {
     if (xpart64.n < (size_t)nxb64 * 3) xpart64.alloc((size_t)nxb64 * 3);
     if (zpart64.n < (size_t)nzb64) zpart64.alloc((size_t)nzb64);
     L("zprep64", [&] { ozd->zprep(uh[cur].p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
-    L("y_inv2_64", [&] { oxd->ypass(P64.p, Q64.p, 2, +1, g, twxd.p, st); });
+    L("y_inv2_64", [&] { oxd->ypass(P64.p, Q64.p, 2, +1, g, twxd.p, nullptr, st); });
     XaArgs<double> a;
     a.P = P64.p;
     a.Q = Q64.p;
@@ -991,7 +999,7 @@ struct Solver final : SolverBase {
     a.kappa = p.kappa;
     a.inv_n = 1.0 / double(g.nr);
     a.rl = Rows{g.ny, g.pitch, g.pp};
-    L("xa64", [&] { oxd->xa(a, false, st); });
+    L("xa64", [&] { oxd->xa(a, XA_PLAIN, st); });
     VB_HIP(hipMemcpyAsync(scal64.p, scal.p, sizeof(Scal), hipMemcpyDeviceToDevice, st));
     L("finalize", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
     check_launch();
@@ -1001,7 +1009,7 @@ struct Solver final : SolverBase {
 
   void query(double* E, double* V, double* A, double* Eb) override {
     if (sizeof(T) == sizeof(double)) {
-      ensure_prefixed();
+      ensure_integrals();
       read_scal();
     } else {
       if (targets_drift()) sync_targets_from_device();
@@ -1034,14 +1042,14 @@ struct Solver final : SolverBase {
     time_step = dt;
     time_current = t;
     set_targets_device();
-    if (prefixed) refinalize(FIN_PLAIN);
+    if (prefixed || prefixed_f) refinalize(FIN_PLAIN);
     started = t > 0.0 || started;
   }
 
   void get_phi(double* host) override {
     VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<double>) * (size_t)g.ncp, hipMemcpyDeviceToDevice, st));
     L("z_inv", [&] { ozd->zpass(S.p, +1, g, twzd.p, st); });
-    L("y_inv", [&] { oxd->ypass(S.p, nullptr, 1, +1, g, twxd.p, st); });
+    L("y_inv", [&] { oxd->ypass(S.p, nullptr, 1, +1, g, twxd.p, nullptr, st); });
     L("x_c2r", [&] { oxd->xc2r_d(S.p, phid.p, npairs, twxd.p, g, st); });
     check_launch();
     VB_HIP(hipMemcpyAsync(host, phid.p, sizeof(double) * (size_t)g.nr, hipMemcpyDeviceToHost, st));
@@ -1053,7 +1061,7 @@ struct Solver final : SolverBase {
     iter_open = false;
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
     L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
-    L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, st); });
+    L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, nullptr, st); });
     L("z_fwd", [&] { ozd->zpass(uh[cur].p, -1, g, twzd.p, st); });
     check_launch();
     VB_HIP(hipStreamSynchronize(st));
@@ -1112,7 +1120,7 @@ size_t solver_footprint(const vb_params& p) {
   const bool mixed = p.precision == VB_FP32_MIXED;
   const size_t cs = mixed ? 8 : 16, rs = cs / 2;
   const bool euler = p.scheme == VB_EULER || p.shape_adjust;
-  const size_t work = 6 + (euler ? 1 : 0);
+  const size_t work = 8 + (euler ? 1 : 0);   // P, Q, Tt, Ts, pre, Na, Nb, Vh (+ W)
   size_t bytes = 3 * nc * 16 + work * nc * cs + (euler ? 2 * nr * rs : 0) + nr * 8;
   if (mixed) bytes += 3 * nc * 16;   // FP64 query buffers
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;

@@ -97,5 +97,7 @@ def test_precision_case_256(vb, gpu):
     assert rel(q32[1], q64[1]) < 1e-7         # V
     assert rel(q32[2], q64[2]) < 1e-7         # A
     # E_b weights the high modes, where the FP32 transforms' rounding lives, by
-    # about eps k^2; E also carries the drifted targets of each run
-    assert rel(q32[3], q64[3]) < 1e-5 and rel(q32[0], q64[0]) < 1e-3
+    # about eps k^2 (up to ~800 here): over the first 100 steps of this case its
+    # difference ranges from 5e-8 to 2e-5 while phi stays within 3e-7; E also
+    # carries the drifted targets of each run
+    assert rel(q32[3], q64[3]) < 3e-5 and rel(q32[0], q64[0]) < 1e-3
