@@ -169,7 +169,9 @@ struct XaArgs {
 // (F(tmp1) + kappa v^)/eps.  This kernel writes the x-forward of tmp1 and s;
 // the stage combine forms N^ from their spectra, so lap(tmp1) never goes back
 // to real space.
-template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false>
+// SUMS: the partial sums only (the integrals of a field whose stage data is
+// not needed, as in the first half of the penalty-drift double step).
+template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false, bool SUMS = false>
 __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
@@ -223,7 +225,9 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
       }
     }
   }
-  if constexpr (WITHS) {   // one transform after the other: fewer live registers
+  if constexpr (SUMS) {
+    // no transform out
+  } else if constexpr (WITHS) {   // one transform after the other: fewer live registers
     {
       cpx<T> z[E];
 #pragma unroll

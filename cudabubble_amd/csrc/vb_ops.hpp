@@ -17,12 +17,13 @@ namespace vb {
 // then issues one launch per recorded position for all solves of a size
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
-  OP_XA, OP_XA_F, OP_XA_S, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
+  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
   OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22,
   OP_ZS41N, OP_ZS42N, OP_ZS43N, OP_ZS44N, OP_ZS21N, OP_ZS22N, OP_FIN
 };
-// x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N)
-enum XaMode { XA_PLAIN = 0, XA_WITHF = 1, XA_WITHS = 2 };
+// x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
+// the partial sums only
+enum XaMode { XA_PLAIN = 0, XA_WITHF = 1, XA_WITHS = 2, XA_SUMS = 3 };
 struct RecLaunch {
   // issues the table form of the recorded kernel over `lanes` solves
   void (*launch_tab)(int op, const void* tab, dim3 grid, dim3 block, int lanes, hipStream_t s);

@@ -44,12 +44,16 @@ struct Launch {
   }
   static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
     const dim3 gr(xgrid(a.npairs)), bl(XWG);
-    if (g_recorder)
-      return record_launch(&launch_tab, mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : OP_XA, a, gr, bl);
+    if (g_recorder) {
+      const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
+      return record_launch(&launch_tab, op, a, gr, bl);
+    }
     if (mode == XA_WITHF)
       hipLaunchKernelGGL((k_xa<T, N, true>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_WITHS)
       hipLaunchKernelGGL((k_xa<T, N, false, false, true>), gr, bl, 0, s, a, nullptr);
+    else if (mode == XA_SUMS)
+      hipLaunchKernelGGL((k_xa<T, N, false, false, false, true>), gr, bl, 0, s, a, nullptr);
     else
       hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
   }
@@ -133,6 +137,9 @@ struct Launch {
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_S:
         hipLaunchKernelGGL((k_xa<T, N, false, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        break;
+      case OP_XA_SUMS:
+        hipLaunchKernelGGL((k_xa<T, N, false, true, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
       case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
       case OP_Y_FWD:

@@ -430,6 +430,28 @@ struct Solver final : SolverBase {
     prefixed_f = withf;
   }
 
+  // The integrals of the prepared stage field alone (no stage data): y-inverse,
+  // the x kernel's partial sums, and the finalize kernel.
+  void prefix_integrals(int mode) {
+    L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, nullptr, st); });
+    XaArgs<T> a{};
+    a.P = P.p;
+    a.Q = Q.p;
+    a.part = xpart.p;
+    a.tw = twx.p;
+    a.npairs = npairs;
+    a.eps = p.eps;
+    a.kappa = p.kappa;
+    a.D = p.D;
+    a.inv_n = 1.0 / double(g.nr);
+    a.rl = Rows{g.ny, g.pitch, g.pp};
+    L("xa_sums", [&] { ox->xa(a, XA_SUMS, st); });
+    refinalize(mode);
+    prepped = false;
+    prefixed = false;
+    prefixed_f = false;
+  }
+
   void refinalize(int mode) {
     L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
     check_launch();
@@ -534,7 +556,7 @@ struct Solver final : SolverBase {
       int out = eval(time_step);
       const int keep = cur;
       cur = out;
-      prefix(FIN_DRIFT, false);
+      prefix_integrals(FIN_DRIFT);   // only the drift of the targets is used
       cur = keep;
       prepped = prefixed = prefixed_f = false;
       ensure_prefixed();
