@@ -21,7 +21,9 @@ namespace vb {
 
 constexpr int XWG = 64;   // threads per x-kernel workgroup: one wave (measured best at 256^3: 64 < 128 < 256)
 
-template <int N> constexpr int xlanes() { return XWG / TFof<N>(); }
+// Plan variant of the x kernels (row transforms).
+template <typename T, int N> constexpr int xplan() { return 0; }
+template <typename T, int N> constexpr int xlanes() { return XWG / TFof<N, xplan<T, N>()>(); }
 
 // Minimum waves per SIMD requested for the fused x kernels: in FP32 four
 // (the register budget that allows it costs at most a few spilled words and
@@ -66,10 +68,10 @@ struct Rows {
 
 // After a forward transform of Z = a + i b (a, b real rows), write the half
 // spectra of a and b, times s.  Uses the lane's LDS; ends with a barrier.
-template <typename T, int N, class Lds>
+template <typename T, int N, int V = 0, class Lds>
 __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, cpx<T>* __restrict__ outA,
                                              cpx<T>* __restrict__ outB, T s, bool valid) {
-  constexpr int E = Plan<N>::E, TF = N / E, NH = N / 2 + 1;
+  constexpr int E = Plan<N, V>::E, TF = N / E, NH = N / 2 + 1;
 #pragma unroll
   for (int e = 0; e < E; ++e) lds(j + TF * e) = x[e];
   __syncthreads();
@@ -95,7 +97,7 @@ template <typename TI, typename T, int N>
 __global__ void __launch_bounds__(XWG) k_xr2c(const TI* __restrict__ in, cpx<T>* __restrict__ out,
                                                long long npairs, const cpx<T>* __restrict__ tw, double scl,
                                                Rows rl) {
-  constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
@@ -109,15 +111,15 @@ __global__ void __launch_bounds__(XWG) k_xr2c(const TI* __restrict__ in, cpx<T>*
     const int p = j + TF * e;
     x[e] = mk<T>(T(in[r0 * N + p]), T(in[(r0 + 1) * N + p]));
   }
-  fft<T, N, -1>(x, j, tw, lds);
+  fft<T, N, -1, XP>(x, j, tw, lds);
   const long long o0 = rl.off(r0);
-  unpack_store<T, N>(x, j, lds, out + o0, out + o0 + rl.pitch, T(scl), valid);
+  unpack_store<T, N, XP>(x, j, lds, out + o0, out + o0 + rl.pitch, T(scl), valid);
 }
 
 template <typename T, typename TO, int N>
 __global__ void __launch_bounds__(XWG) k_xc2r(const cpx<T>* __restrict__ in, TO* __restrict__ out,
                                                long long npairs, const cpx<T>* __restrict__ tw, Rows rl) {
-  constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__(XWG) k_xc2r(const cpx<T>* __restrict__ in, TO*
   const long long o0 = rl.off(r0);
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(in + o0, in + o0 + rl.pitch, j + TF * e);
-  fft<T, N, +1>(x, j, tw, lds);
+  fft<T, N, +1, XP>(x, j, tw, lds);
   if (valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -174,7 +176,7 @@ struct XaArgs {
 template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false, bool SUMS = false>
 __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
   __shared__ double red[3 * (XWG / 64)];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
       for (int e = 0; e < E; ++e)
         x[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.Q + a.rl.off(r0 + h), j + TF * e);
     }
-    fft2<T, N, +1>(x[0], x[1], j, a.tw, lds, lds2);
+    fft2<T, N, +1, XP>(x[0], x[1], j, a.tw, lds, lds2);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const long long r = r0 + h;
@@ -232,27 +234,27 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
       cpx<T> z[E];
 #pragma unroll
       for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
-      fft<T, N, -1>(z, j, a.tw, lds);
-      unpack_store<T, N>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+      fft<T, N, -1, XP>(z, j, a.tw, lds);
+      unpack_store<T, N, XP>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
     }
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(ss[0][e], ss[1][e]);
-    fft<T, N, -1>(z, j, a.tw, lds2);
-    unpack_store<T, N>(z, j, lds2, a.Ts + a.rl.off(r0), a.Ts + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    fft<T, N, -1, XP>(z, j, a.tw, lds2);
+    unpack_store<T, N, XP>(z, j, lds2, a.Ts + a.rl.off(r0), a.Ts + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   } else {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
-    fft<T, N, -1>(z, j, a.tw, lds);
-    unpack_store<T, N>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    fft<T, N, -1, XP>(z, j, a.tw, lds);
+    unpack_store<T, N, XP>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   if constexpr (WITHF) {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(ff[0][e], ff[1][e]);
-    fft<T, N, -1>(z, j, a.tw, lds);
-    unpack_store<T, N>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    fft<T, N, -1, XP>(z, j, a.tw, lds);
+    unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
@@ -275,7 +277,7 @@ struct XbArgs {
 template <typename T, int N, bool TAB = false>
 __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbArgs<T>* tab) {
   const XbArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
@@ -293,8 +295,8 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbA
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, j + TF * e);
-  fft2<T, N, +1>(vq[0], vq[1], j, a.tw, lds, lds2);   // v + i lap v of rows r0, r0 + 1
-  fft<T, N, +1>(x, j, a.tw, lds);                     // lap tmp1 of rows r0 (re), r0 + 1 (im)
+  fft2<T, N, +1, XP>(vq[0], vq[1], j, a.tw, lds, lds2);   // v + i lap v of rows r0, r0 + 1
+  fft<T, N, +1, XP>(x, j, a.tw, lds);                     // lap tmp1 of rows r0 (re), r0 + 1 (im)
   const double l1 = a.sc->l1, l2 = a.sc->l2;
   const double eps = a.eps, ie = 1.0 / eps, ie2 = ie * ie, ie3 = ie * ie2;
   const double D = a.D, kappa = a.kappa, eta = a.eta;
@@ -313,8 +315,8 @@ __global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbA
     }
     x[e] = mk<T>(T(nn[0]), T(nn[1]));
   }
-  fft<T, N, -1>(x, j, a.tw, lds);
-  unpack_store<T, N>(x, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+  fft<T, N, -1, XP>(x, j, a.tw, lds);
+  unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,7 +339,7 @@ struct XeArgs {
 
 template <typename T, int N>
 __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
-  constexpr int E = Plan<N>::E, TF = N / E, G = xlanes<N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[G * LL];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
@@ -348,7 +350,7 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
   cpx<T> x[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, j + TF * e);
-  fft<T, N, +1>(x, j, a.tw, lds);
+  fft<T, N, +1, XP>(x, j, a.tw, lds);
   const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.t, ie2 = 1.0 / (eps * eps);
   double acc[1] = {0.0};
 #pragma unroll
@@ -372,8 +374,8 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
     block_sum_store<1, XWG>(acc, red, a.part + blockIdx.x);
     return;
   }
-  fft<T, N, -1>(x, j, a.tw, lds);
-  unpack_store<T, N>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+  fft<T, N, -1, XP>(x, j, a.tw, lds);
+  unpack_store<T, N, XP>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------

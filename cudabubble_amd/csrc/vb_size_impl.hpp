@@ -11,7 +11,7 @@ namespace vb {
 template <typename T, int N>
 struct Launch {
   static constexpr int TF = TFof<N>();
-  static constexpr int G = xlanes<N>();
+  static constexpr int G = xlanes<T, N>();
   static constexpr int TB = tbw<T, N>();
   static constexpr int YT = TB * TF;
   static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
