@@ -5,15 +5,17 @@
 //
 //   z-kernel  : v^ -> z-inverse of v^ and of -|k|^2 v^   (+ Parseval |grad v|^2)
 //   y-kernel  : y-inverse of both
-//   xa-kernel : x-inverse (two-for-one) -> v, lap v; f, tmp1, sums of v,
-//               (v^2-1)^2, f^2; x-forward of tmp1            (P1 fused)
-//   y, z(lap), y kernels : lap tmp1 spectrum round trip; extra workgroups
-//               of the z(lap) launch evaluate V, A, E_b, lambda1, lambda2, E
-//               from the block partials (finalize_body)
-//   xb-kernel : x-inverse -> lap tmp1; nld, N; x-forward of N  (P2 fused)
-//   y-kernel  : y-forward of N
-//   zstage    : z-forward -> N^, ETD combine with the state, and the first
-//               kernel of the next stage (z-inverse of the next field)
+//   xa-kernel : x-inverse (two-for-one) -> v, lap v; f, tmp1, s, sums of v,
+//               (v^2-1)^2, f^2; x-forward of tmp1 and s
+//   y-kernel  : y-forward of both; extra workgroups of the launch evaluate
+//               V, A, E_b, lambda1, lambda2, E from the block partials
+//               (finalize_body)
+//   zstage    : z-forward of both -> N^ in spectral space (k_xa explains
+//               why that is exact), the ETD combine with the state, and the
+//               first kernel of the next stage (z-inverse of the next field)
+//
+// The Euler paths keep a real-space right-hand side: xa also stores v and f
+// and transforms f, and y, z(lap), y kernels take lap f back to x-spectra.
 #pragma once
 #include "vb_device.hpp"
 
@@ -27,11 +29,13 @@ template <typename T, int N> constexpr int xlanes() { return XWG / TFof<N, xplan
 
 // Minimum waves per SIMD requested for the fused x kernels: in FP32 four
 // (the register budget that allows it costs at most a few spilled words and
-// gains more in latency hiding than it loses); FP64 keeps the full budget.
+// gains more in latency hiding than it loses); FP64, and the FP32 x kernel of
+// the ETD stages (two forward transforms out, about 200 registers; a cap of
+// three or four spills and ran 33 % slower), keep the full budget.
 template <typename T, bool WITHS = false> constexpr int xwaves() { return (sizeof(T) == 4 && !WITHS) ? 4 : 1; }
 
-// Plan variant of the FP32 column kernels (y passes, z Laplacian, stage-3
-// combine) at 256: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
+// Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256:
+// 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
 template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && N == 256) ? 1 : 0; }
 
 // Column tile width of the y / z kernels (columns that are contiguous in
@@ -216,7 +220,7 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
         const double t1 = v * (v * v - 1.0 - kappa);       // nld_kernel1, Calc.cu:270
         if (valid) {
           acc[0] += v; acc[1] += w * w; acc[2] += f * f;
-          if constexpr (WITHF) {   // the Euler update reads v and f back; xb recomputes them
+          if constexpr (WITHF) {   // the Euler update reads v and f back
             a.v[r * N + p] = T(v);
             a.f[r * N + p] = T(f);
           }
@@ -257,66 +261,6 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
     unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
-}
-
-// ---------------------------------------------------------------------------
-// xb: stage suffix, P2 fused
-// ---------------------------------------------------------------------------
-template <typename T>
-struct XbArgs {
-  cpx<T>* Tt;        // in: x-half-spectra of lap tmp1; out: x-forward of N, times inv_n
-  const cpx<T>* P;   // x-half-spectra of v and lap v, as xa read them: v and f are
-  const cpx<T>* Q;   // recomputed here instead of being stored by xa and read back
-  const Scal* sc;
-  const cpx<T>* tw;
-  long long npairs;
-  double eps, kappa, D, eta, inv_n;
-  Rows rl;
-};
-
-template <typename T, int N, bool TAB = false>
-__global__ void __launch_bounds__(XWG, xwaves<T>()) k_xb(XbArgs<T> a0, const XbArgs<T>* tab) {
-  const XbArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
-  __shared__ cpx<T> sm[2 * G * LL];
-  const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
-  RowLds<T> lds{sm + lane * LL};
-  RowLds<T> lds2{sm + (G + lane) * LL};
-  long long pair = (long long)blockIdx.x * G + lane;
-  const bool valid = pair < a.npairs;
-  if (!valid) pair = a.npairs - 1;
-  const long long r0 = 2 * pair;
-  cpx<T> x[E], vq[2][E];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-      vq[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.Q + a.rl.off(r0 + h), j + TF * e);
-  }
-#pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, j + TF * e);
-  fft2<T, N, +1, XP>(vq[0], vq[1], j, a.tw, lds, lds2);   // v + i lap v of rows r0, r0 + 1
-  fft<T, N, +1, XP>(x, j, a.tw, lds);                     // lap tmp1 of rows r0 (re), r0 + 1 (im)
-  const double l1 = a.sc->l1, l2 = a.sc->l2;
-  const double eps = a.eps, ie = 1.0 / eps, ie2 = ie * ie, ie3 = ie * ie2;
-  const double D = a.D, kappa = a.kappa, eta = a.eta;
-  const double c32 = 3.0 / (2.0 * sqrt(2.0));   // multiplied, not divided (see k_xa)
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    double nn[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const double dt1 = double(h ? x[e].y : x[e].x);
-      const double v = double(vq[h][e].x), lv = double(vq[h][e].y);
-      const double f = -lv * eps + (v * v - 1.0) * v * ie;                           // Calc.cu:160
-      const double t1 = v * (v * v - 1.0 - kappa);
-      const double nl = -dt1 * ie + D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f;   // Calc.cu:282
-      nn[h] = -l1 * 0.5 - l2 * f * c32 - nl * eta;                                 // Calc.cu:435
-    }
-    x[e] = mk<T>(T(nn[0]), T(nn[1]));
-  }
-  fft<T, N, -1, XP>(x, j, a.tw, lds);
-  unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -681,12 +625,12 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
 
 template <typename T>
 struct ZsArgs {
-  const cpx<T>* U;        // x,y-forward of N (normalised); NF: of tmp1
-  const cpx<T>* U2;       // NF: x,y-forward of s
-  cpx<T>* Vh;             // NF: spectrum of the stage field (stages after the
-                          // first) and of the next one, group-major, in place
-  const Scal* sc;         // NF: lambda1, lambda2 of the stage field
-  double eps, kappa, eta; // NF
+  const cpx<T>* U;        // x,y-forward of tmp1 (normalised)
+  const cpx<T>* U2;       // x,y-forward of s (normalised)
+  cpx<T>* Vh;             // spectrum of the stage field (stages after the first)
+                          // and of the next one, group-major, in place
+  const Scal* sc;         // lambda1, lambda2 of the stage field
+  double eps, kappa, eta;
   const cpx<double>* uh;  // state spectrum u^
   cpx<T>* pre;            // ETDRK4: e1 a^ - q N^u (stage 1 -> 3)
   cpx<T>* Na;             // ETDRK4: N^a (stage 2 -> 4); ETDRK2: N^u (stage 1 -> 2)
@@ -704,35 +648,18 @@ struct ZsArgs {
 
 // SCH = 4 (ETDRK4, belastic.cpp:519-552 / Calc.cu:488-577) or 2 (ETDRK2,
 // belastic.cpp:491-517 / Calc.cu:446-486); STG = stage whose N^ this is.
-// The N^ of a stage leaves the FFT in precision T and is stored as such, so
-// keeping it for a later stage loses nothing; the state and every combine
-// that produces it are FP64.
-// Minimum waves per SIMD for the stage combines: the FP32 ETDRK4 stage 1
-// asks for four (at most a few spilled words); stage 2 fits four without
-// asking; stage 4 (FP64 state and two stage spectra live across its
-// transform) spills over 30 registers at four and runs slower than at three.
-template <typename T, int SCH, int STG, bool NF = false> constexpr int zwaves() {
-  return (sizeof(T) == 4 && SCH == 4 && STG == 1 && !NF) ? 4 : 1;
-}
-// whether a stage combine loads its other operands before its forward
-// transform (overlapping them with it) or after it (fewer live registers);
-// every stage measured faster with the loads first
-template <typename T, int SCH, int STG, bool NF = false> constexpr bool zprefetch() { return !NF; }
-// plan variant of a stage combine: 16 x 16 pays for stage 3 at 256 (FP32)
-template <typename T, int N, int SCH, int STG, bool NF = false> constexpr int zsplan() {
-  return (SCH == 4 && STG == 3 && !NF) ? colplan<T, N>() : 0;
-}
-
-// NF: the stage's N^ is formed here from the spectra of tmp1 and s (see
-// k_xa), the stage field's spectrum v^ (the state u^ in the first stage, Vh
-// after it) and lambda1, lambda2; the stage also stores the next stage
-// field's spectrum in Vh.
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, bool NF = false,
-          int V = zsplan<T, N, SCH, STG, NF>()>
-__global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG, NF>())) k_zstage(ZsArgs<T> a0,
-                                                                                     const ZsArgs<T>* tab) {
+// The stage's N^ is formed here from the spectra of tmp1 and s (see k_xa),
+// the stage field's spectrum v^ (the state u^ in the first stage, Vh after
+// it) and lambda1, lambda2; the stage also stores the next stage field's
+// spectrum in Vh.  The N^ of a stage is stored in precision T for a later
+// stage; the state and every combine that produces it are FP64.
+// All stages load their operands after the forward transforms, and ask for
+// no minimum occupancy: a cap of three or four waves per SIMD spills heavily
+// and runs slower.
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false>
+__global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int E = Plan<N, V>::E, TF = N / E;
+  constexpr int E = Plan<N>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
@@ -741,62 +668,52 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG, NF>(
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g);
   const long long plane = g.pp;
-  // pre, Na and Nb live only between the stages of one evaluation, so they
-  // are kept group-major: this workgroup's TB columns of all N planes are one
-  // contiguous run (see zs_private), which keeps the strided z traffic of a
-  // stage to the spectra that other kernels share
+  // pre, Na, Nb and Vh live only between the stages of one evaluation, so
+  // they are kept group-major: this workgroup's TB columns of all N planes
+  // are one contiguous run (see zs_private), which keeps the strided z
+  // traffic of a stage to the spectra that other kernels share
   const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
-  cpx<T> x[E], x2[NF ? E : 1];
+  cpx<T> x[E], x2[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
-    if constexpr (NF) x2[e] = a.U2[id.q + (long long)(j + TF * e) * plane];
+    x2[e] = a.U2[id.q + (long long)(j + TF * e) * plane];
+  }
+  // Calc.cu:428-444 with nld (Calc.cu:265-297) moved to spectral space:
+  // N^ = -lambda1/2 delta_k0 - lambda2 3/(2 sqrt 2) F(f) - eta F(s)
+  //      - (eta/eps) |k|^2 F(tmp1),  F(f) = eps |k|^2 v^ + (F(tmp1) + kappa v^)/eps.
+  // Everything but the v^ terms is formed right after the transforms, so the
+  // spectrum of s is dead before the combine's operands load:
+  // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
+  fft2<T, N, -1>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
+  const double l1h = 0.5 * a.sc->l1;
+  const double l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
+  const double ie = 1.0 / a.eps;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int kz = j + TF * e;
+    const double k2 = double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
+    const double at = -(l2c + a.eta * k2) * ie;   // on F(tmp1)
+    double pr = at * double(x[e].x) - a.eta * double(x2[e].x);
+    const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
+    if (id.q == 0 && kz == 0) pr -= l1h;
+    x[e] = mk<T>(T(pr), T(pi));
   }
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
-  constexpr bool READS_V = NF && STG > 1;   // stage field spectrum from Vh
   cpx<double> ur[USES_U ? E : 1];
   cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? E : 1];
   cpx<T> pb[(SCH == 4 && STG == 4) ? E : 1];
-  cpx<T> pv[READS_V ? E : 1];
-  auto load_operands = [&]() {
+  cpx<T> pv[STG > 1 ? E : 1];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const long long i = id.q + (long long)(j + TF * e) * plane;
-      const long long ib = ib0 + (long long)(j + TF * e) * TB;
-      if constexpr (USES_U) ur[e] = a.uh[i];
-      if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[ib];
-      if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[ib]; pb[e] = a.Nb[ib]; }
-      if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[ib];
-      if constexpr (READS_V) pv[e] = a.Vh[ib];
-    }
-  };
-  if constexpr (zprefetch<T, SCH, STG, NF>()) load_operands();
-  double l2c = 0.0, ie = 0.0;
-  if constexpr (NF) {
-    // Calc.cu:428-444 with nld (Calc.cu:265-297) moved to spectral space:
-    // N^ = -lambda1/2 delta_k0 - lambda2 3/(2 sqrt 2) F(f) - eta F(s)
-    //      - (eta/eps) |k|^2 F(tmp1),  F(f) = eps |k|^2 v^ + (F(tmp1) + kappa v^)/eps.
-    // Everything but the v^ terms is formed right after the transforms, so
-    // the spectrum of s is dead before the combine's operands load:
-    // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
-    fft2<T, N, -1, V>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
-    const double l1h = 0.5 * a.sc->l1;
-    l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
-    ie = 1.0 / a.eps;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int kz = j + TF * e;
-      const double k2 = double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
-      const double at = -(l2c + a.eta * k2) * ie;   // on F(tmp1)
-      double pr = at * double(x[e].x) - a.eta * double(x2[e].x);
-      const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
-      if (id.q == 0 && kz == 0) pr -= l1h;
-      x[e] = mk<T>(T(pr), T(pi));
-    }
-  } else {
-    fft<T, N, -1, V>(x, j, a.tw, lds);              // N^ of this stage
+  for (int e = 0; e < E; ++e) {
+    const long long i = id.q + (long long)(j + TF * e) * plane;
+    const long long ib = ib0 + (long long)(j + TF * e) * TB;
+    if constexpr (USES_U) ur[e] = a.uh[i];
+    if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[ib];
+    if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[ib]; pb[e] = a.Nb[ib]; }
+    if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[ib];
+    if constexpr (STG > 1) pv[e] = a.Vh[ib];
   }
-  if constexpr (!zprefetch<T, SCH, STG, NF>()) load_operands();
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll
@@ -805,25 +722,17 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG, NF>(
     const long long i = id.q + (long long)kz * plane;
     const long long ib = ib0 + (long long)kz * TB;
     const int m = mindex(g, id.kx, id.ky, kz);
-    double nr, ni;   // N^ of this stage
-    if constexpr (NF) {
-      const double k2 = double(m) * g.kfac2;
-      const double av = -l2c * (a.eps * k2 + a.kappa * ie);
-      double vr, vi;
-      if constexpr (STG == 1) {
-        vr = ur[e].x;
-        vi = ur[e].y;
-      } else {
-        vr = double(pv[e].x);
-        vi = double(pv[e].y);
-      }
-      nr = double(x[e].x) + av * vr;
-      ni = double(x[e].y) + av * vi;
-      x[e] = mk<T>(T(nr), T(ni));
+    const double av = -l2c * (a.eps * double(m) * g.kfac2 + a.kappa * ie);
+    double vr, vi;   // the stage field's spectrum
+    if constexpr (STG == 1) {
+      vr = ur[e].x;
+      vi = ur[e].y;
     } else {
-      nr = double(x[e].x);
-      ni = double(x[e].y);
+      vr = double(pv[e].x);
+      vi = double(pv[e].y);
     }
+    const double nr = double(x[e].x) + av * vr, ni = double(x[e].y) + av * vi;   // N^ of this stage
+    x[e] = mk<T>(T(nr), T(ni));
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
       const double2 c = a.c1[m];                      // e1, q
@@ -868,14 +777,14 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zwaves<T, SCH, STG, NF>(
       fi = ai + c2 * (ni - double(nu.y));
       if (id.valid) a.uout[i] = mk<double>(fr, fi);
     }
-    if constexpr (NF && STG < SCH) {
+    if constexpr (STG < SCH) {
       if (id.valid) a.Vh[ib] = mk<T>(T(fr), T(fi));
     }
     if (e == 0) dc0 = fr;
     next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
-  next_field_tail<T, N, TB, V>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB, 0>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
 // widening copy of a spectrum into the FP64 state

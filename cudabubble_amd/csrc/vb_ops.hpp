@@ -17,9 +17,8 @@ namespace vb {
 // then issues one launch per recorded position for all solves of a size
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
-  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_XB, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
-  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22,
-  OP_ZS41N, OP_ZS42N, OP_ZS43N, OP_ZS44N, OP_ZS21N, OP_ZS22N, OP_FIN
+  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN
 };
 // x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
 // the partial sums only
@@ -49,7 +48,6 @@ inline void record_launch(void (*tabfn)(int, const void*, dim3, dim3, int, hipSt
 }
 
 template <typename T> struct XaArgs;
-template <typename T> struct XbArgs;
 template <typename T> struct XeArgs;
 template <typename T> struct ZsArgs;
 
@@ -66,7 +64,6 @@ struct SizeOps {
   void (*xc2r_d)(const cpx<T>* in, double* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
   void (*xc2r)(const cpx<T>* in, T* out, long long npairs, const cpx<T>* tw, const Geo& g, hipStream_t s);
   void (*xa)(const XaArgs<T>& a, int mode, hipStream_t s);   // mode: XaMode
-  void (*xb)(const XbArgs<T>& a, hipStream_t s);
   void (*xe)(const XeArgs<T>& a, hipStream_t s);
   // fin (or nullptr): integrals evaluated by extra workgroups of the launch
   void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, const FinArgs* fin,
@@ -76,8 +73,7 @@ struct SizeOps {
   void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s);
   void (*zprep)(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g, const cpx<T>* tw,
                 hipStream_t s);
-  // nf: N^ formed in the combine from the spectra of tmp1 and s (k_zstage)
-  void (*zstage)(const ZsArgs<T>& a, int sch, int stg, bool nf, hipStream_t s);
+  void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);
 };
 
 template <typename T, int N> const SizeOps<T>* ops_for();

@@ -17,9 +17,6 @@ struct Launch {
   static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
   static constexpr int CB = colw<T, N>();             // their columns per workgroup
   static constexpr int CT = CB * TFof<N, CV>();       // and threads per workgroup
-  template <int SCH, int STG, bool NF = false> static constexpr int zs_threads() {
-    return TB * TFof<N, zsplan<T, N, SCH, STG, NF>()>();
-  }
   // workgroups of the integrals in a launch that evaluates them (nullptr: none)
   static int fin_groups(const FinArgs* fin) {
     return fin ? std::min(kFinShares, std::max(1, (fin->nxb + 1023) / 1024)) : 0;
@@ -57,11 +54,6 @@ struct Launch {
     else
       hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
   }
-  static void xb(const XbArgs<T>& a, hipStream_t s) {
-    const dim3 gr(xgrid(a.npairs)), bl(XWG);
-    if (g_recorder) return record_launch(&launch_tab, OP_XB, a, gr, bl);
-    hipLaunchKernelGGL((k_xb<T, N>), gr, bl, 0, s, a, nullptr);
-  }
   static void xe(const XeArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
   }
@@ -97,36 +89,29 @@ struct Launch {
     if (g_recorder) return record_launch(&launch_tab, OP_ZPREP, a, gr, bl);
     hipLaunchKernelGGL((k_zprep<T, N>), gr, bl, 0, s, a, nullptr);
   }
-  template <int SCH, int STG, bool TAB, bool NF>
+  template <int SCH, int STG, bool TAB>
   static void zs_launch(dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
-    const dim3 bl((unsigned)zs_threads<SCH, STG, NF>());
-    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB, NF>), gr, bl, 0, s, a, tab);
+    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB>), gr, dim3(YT), 0, s, a, tab);
   }
-  template <bool TAB, bool NF>
+  template <bool TAB>
   static void zs_dispatch(int sch, int stg, dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
     if (sch == 4) {
       switch (stg) {
-        case 1: zs_launch<4, 1, TAB, NF>(gr, a, tab, s); break;
-        case 2: zs_launch<4, 2, TAB, NF>(gr, a, tab, s); break;
-        case 3: zs_launch<4, 3, TAB, NF>(gr, a, tab, s); break;
-        default: zs_launch<4, 4, TAB, NF>(gr, a, tab, s); break;
+        case 1: zs_launch<4, 1, TAB>(gr, a, tab, s); break;
+        case 2: zs_launch<4, 2, TAB>(gr, a, tab, s); break;
+        case 3: zs_launch<4, 3, TAB>(gr, a, tab, s); break;
+        default: zs_launch<4, 4, TAB>(gr, a, tab, s); break;
       }
     } else if (stg == 1) {
-      zs_launch<2, 1, TAB, NF>(gr, a, tab, s);
+      zs_launch<2, 1, TAB>(gr, a, tab, s);
     } else {
-      zs_launch<2, 2, TAB, NF>(gr, a, tab, s);
+      zs_launch<2, 2, TAB>(gr, a, tab, s);
     }
   }
-  static void zstage(const ZsArgs<T>& a, int sch, int stg, bool nf, hipStream_t s) {
+  static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
     const dim3 gr(zgrid(a.g));
-    if (g_recorder) {
-      const int op = (nf ? (sch == 4 ? OP_ZS41N : OP_ZS21N) : (sch == 4 ? OP_ZS41 : OP_ZS21)) + (stg - 1);
-      return record_launch(&launch_tab, op, a, gr, dim3(1));   // the table launch picks the block
-    }
-    if (nf)
-      zs_dispatch<false, true>(sch, stg, gr, a, nullptr, s);
-    else
-      zs_dispatch<false, false>(sch, stg, gr, a, nullptr, s);
+    if (g_recorder) return record_launch(&launch_tab, (sch == 4 ? OP_ZS41 : OP_ZS21) + (stg - 1), a, gr, dim3(YT));
+    zs_dispatch<false>(sch, stg, gr, a, nullptr, s);
   }
 
   // the table form of a recorded launch, over `lanes` solves (blockIdx.z)
@@ -141,7 +126,6 @@ struct Launch {
       case OP_XA_SUMS:
         hipLaunchKernelGGL((k_xa<T, N, false, true, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
-      case OP_XB: hipLaunchKernelGGL((k_xb<T, N, true>), gr, bl, 0, s, XbArgs<T>{}, (const XbArgs<T>*)tab); break;
       case OP_Y_FWD:
         hipLaunchKernelGGL((k_ypass<T, N, -1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
@@ -155,16 +139,10 @@ struct Launch {
         hipLaunchKernelGGL((k_zprep<T, N, TB, true>), gr, bl, 0, s, ZprArgs<T>{}, (const ZprArgs<T>*)tab);
         break;
       case OP_ZS41: case OP_ZS42: case OP_ZS43: case OP_ZS44:
-        zs_dispatch<true, false>(4, op - OP_ZS41 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
+        zs_dispatch<true>(4, op - OP_ZS41 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
       case OP_ZS21: case OP_ZS22:
-        zs_dispatch<true, false>(2, op - OP_ZS21 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
-        break;
-      case OP_ZS41N: case OP_ZS42N: case OP_ZS43N: case OP_ZS44N:
-        zs_dispatch<true, true>(4, op - OP_ZS41N + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
-        break;
-      case OP_ZS21N: case OP_ZS22N:
-        zs_dispatch<true, true>(2, op - OP_ZS21N + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
+        zs_dispatch<true>(2, op - OP_ZS21 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
       default: break;
     }
@@ -182,7 +160,6 @@ struct Launch {
     o.xc2r_d = &xc2r_d;
     o.xc2r = &xc2r;
     o.xa = &xa;
-    o.xb = &xb;
     o.xe = &xe;
     o.ypass = &ypass;
     o.zpass = &zpass;

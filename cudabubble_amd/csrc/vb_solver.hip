@@ -215,7 +215,7 @@ struct Solver final : SolverBase {
     for (auto* b : {&P, &Q, &Tt, &Ts}) b->alloc(nc);
     for (auto* b : {&pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
     // W, v and f carry the Euler right-hand side only (run_static, shape
-    // adjust, the residual); the ETD pipeline recomputes v and f in xb
+    // adjust, the residual); the ETD pipeline needs no real fields stored
     if (p.scheme == VB_EULER || p.shape_adjust) {
       W.alloc(nc);
       v.alloc(nr);
@@ -482,7 +482,7 @@ struct Solver final : SolverBase {
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
-    L(names[stg], [&] { oz->zstage(z, sch, stg, true, st); });
+    L(names[stg], [&] { oz->zstage(z, sch, stg, st); });
     check_launch();
     prefixed = false;
     prefixed_f = false;
