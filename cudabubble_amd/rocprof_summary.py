@@ -29,17 +29,25 @@ COPY_BYTES = 256 * 1024 * 1024     # bench.py --copy-calibration: bytes read and
 
 def solver_name(mangled):
     """Maps a demangled rocprof kernel name to (solver kernel name, precision) or None."""
-    m = re.search(r"vb::k_(\w+)<(double|float)(?:, ?(-?\d+))?(?:, ?(-?\d+|true|false))?(?:, ?(-?\d+))?(?:, ?(-?\d+))?", mangled)
+    if "k_finalize" in mangled:
+        return "finalize", None
+    if "k_fill_scal" in mangled:
+        return "fill_scal", None
+    m = re.search(r"vb::k_(\w+)<([^>]*)>", mangled)
     if not m:
-        if "k_finalize" in mangled:
-            return "finalize", None
-        if "k_fill_scal" in mangled:
-            return "fill_scal", None
         return None
-    kind, prec = m.group(1), ("fp64" if m.group(2) == "double" else "fp32")
-    a = [m.group(i) for i in range(3, 7)]
-    if kind == "xa":
-        return ("xa_f" if a[1] == "true" else "xa"), prec
+    kind = m.group(1)
+    args = [x.strip() for x in m.group(2).split(",")]
+    if args[0] not in ("double", "float"):
+        return None
+    prec = "fp64" if args[0] == "double" else "fp32"
+    a = args[1:] + [None] * 6
+    if kind == "xa":   # <T, N, WITHF, TAB, WITHS, SUMS>
+        if a[1] == "true":
+            return "xa_f", prec
+        if a[4] == "true":
+            return "xa_sums", prec
+        return ("xa" if a[3] == "true" else "xa_plain"), prec
     if kind == "ypass":
         return ("y_fwd" if a[1] == "-1" else "y_inv"), prec
     if kind == "zpass":
