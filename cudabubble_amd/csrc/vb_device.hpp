@@ -79,6 +79,16 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double* red, d
   }
 }
 
+// E, Eu and the multipliers from the integrals V, A, E_b and the targets
+// alpha, beta of c (belastic.cpp:68-86 and 473-479).
+__device__ __forceinline__ void apply_targets(Scal& c, const Phys& ph) {
+  const double ve = c.V - c.alpha, se = c.A - c.beta;
+  c.Eu = c.Eb * ph.eta;
+  c.E = c.Eb * ph.eta + 0.5 * ph.M1 * ve * ve + 0.5 * ph.M2 * se * se;
+  c.l1 = ph.M1 * ve;
+  c.l2 = ph.M2 * se;
+}
+
 // Workgroup shares of a split evaluation of the integrals and the ticket that
 // picks the workgroup which adds them; one per solve, zeroed at creation and
 // left zeroed by every evaluation.
@@ -159,7 +169,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, dou
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    Scal& c = *a.sc;
+    Scal c = *a.sc;
     const double N = double(a.g.nr);
     const double L3 = a.g.L3;
     const double eps = a.ph.eps;
@@ -171,26 +181,20 @@ __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, dou
     const double ms = 0.5 * eps * tot[3] + 0.25 / eps * (tot[1] / N);
     const double A = ms * (3.0 / (2.0 * sqrt(2.0)) * L3);
     const double Eb = (tot[2] / N) * (1.0 / (eps * 2) * L3);
-    double alpha = c.alpha, beta = c.beta;
     if (a.mode == FIN_DRIFT) {
-      alpha -= (V - c.alpha0);
-      beta -= (A - c.beta0);
-      c.alpha = alpha;
-      c.beta = beta;
+      c.alpha -= (V - c.alpha0);
+      c.beta -= (A - c.beta0);
     }
-    const double ve = V - alpha, se = A - beta;
     c.V = V;
     c.A = A;
     c.Eb = Eb;
-    c.Eu = Eb * a.ph.eta;
-    c.E = Eb * a.ph.eta + 0.5 * a.ph.M1 * ve * ve + 0.5 * a.ph.M2 * se * se;
-    c.l1 = a.ph.M1 * ve;
-    c.l2 = a.ph.M2 * se;
     c.sum_v = tot[0];
     c.sum_w = tot[1];
     c.sum_f2 = tot[2];
     c.grad2 = tot[3];
     c.nonfinite = !(isfinite(V) && isfinite(A) && isfinite(Eb)) ? 1 : 0;
+    apply_targets(c, a.ph);
+    *a.sc = c;
   }
 }
 

@@ -18,7 +18,7 @@ namespace vb {
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
   OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
-  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET
 };
 // x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
 // the partial sums only

@@ -38,6 +38,54 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a0, const FinA
   finalize_body<kFinThreads>(a, red, tot, 0, 1);
 }
 
+// Across the penalty-drift double step: KEEP saves the scalars of the
+// current state (its integrals do not depend on the targets); RETARGET puts
+// them back under the current (drifted) targets, which is what the finalize
+// kernel would write for that state's partial sums.
+struct KeepArgs {
+  Scal* sc;
+  Scal* keep;
+  Phys ph;
+};
+
+template <bool TAB>
+__global__ void k_keep_scal(KeepArgs a0, const KeepArgs* tab) {
+  const KeepArgs a = TAB ? tab[blockIdx.z] : a0;
+  if (threadIdx.x == 0 && blockIdx.x == 0) *a.keep = *a.sc;
+}
+
+template <bool TAB>
+__global__ void k_retarget(KeepArgs a0, const KeepArgs* tab) {
+  const KeepArgs a = TAB ? tab[blockIdx.z] : a0;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    Scal c = *a.keep;
+    c.alpha = a.sc->alpha;
+    c.beta = a.sc->beta;
+    c.alpha0 = a.sc->alpha0;
+    c.beta0 = a.sc->beta0;
+    apply_targets(c, a.ph);
+    *a.sc = c;
+  }
+}
+
+static void keep_tab(int op, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
+  gr.z = unsigned(lanes);
+  if (op == OP_KEEP)
+    hipLaunchKernelGGL(k_keep_scal<true>, gr, bl, 0, s, KeepArgs{}, (const KeepArgs*)tab);
+  else
+    hipLaunchKernelGGL(k_retarget<true>, gr, bl, 0, s, KeepArgs{}, (const KeepArgs*)tab);
+}
+
+// op: OP_KEEP or OP_RETARGET
+static void launch_keep(int op, Scal* sc, Scal* keep, const Phys& ph, hipStream_t s) {
+  const KeepArgs a{sc, keep, ph};
+  if (g_recorder) return record_launch(&keep_tab, op, a, dim3(1), dim3(64));
+  if (op == OP_KEEP)
+    hipLaunchKernelGGL(k_keep_scal<false>, dim3(1), dim3(64), 0, s, a, nullptr);
+  else
+    hipLaunchKernelGGL(k_retarget<false>, dim3(1), dim3(64), 0, s, a, nullptr);
+}
+
 __global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     sc->alpha = alpha;
@@ -107,11 +155,13 @@ struct Solver final : SolverBase {
   const SizeOps<double>* oxd = nullptr;
   const SizeOps<double>* ozd = nullptr;
   DBuf<cpx<double>> uh[2], S;            // state (FP64) and a transform scratch
-  DBuf<cpx<T>> P, Q, Tt, Ts, W;          // transform work buffers
+  DBuf<cpx<T>> P, Q, Tt, Ts, W;          // transform work buffers; Tt, Ts: the state's prefix
+  DBuf<cpx<T>> Tt1, Ts1;                 // the prefixes of stages 2..4
   DBuf<cpx<T>> pre, Na, Nb, Vh;          // stage spectra kept for later stages (group-major)
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
   DBuf<Scal> scal;
+  DBuf<Scal> scal_keep;                  // the state's integrals across the drift double step
   DBuf<FinScratch> fin_scratch;
   DBuf<cpx<T>> twx, twz;
   DBuf<cpx<double>> twxd, twzd;
@@ -212,7 +262,7 @@ struct Solver final : SolverBase {
     }
     const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
     for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
-    for (auto* b : {&P, &Q, &Tt, &Ts}) b->alloc(nc);
+    for (auto* b : {&P, &Q, &Tt, &Ts, &Tt1, &Ts1}) b->alloc(nc);
     for (auto* b : {&pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
     // W, v and f carry the Euler right-hand side only (run_static, shape
     // adjust, the residual); the ETD pipeline needs no real fields stored
@@ -229,6 +279,7 @@ struct Solver final : SolverBase {
     xpart.alloc((size_t)nxb * 3);
     zpart.alloc((size_t)nzb);
     scal.alloc(1);
+    scal_keep.alloc(1);
     fin_scratch.alloc(1);
     VB_HIP(hipMemsetAsync(fin_scratch.p, 0, sizeof(FinScratch), st));
     VB_HIP(hipHostMalloc((void**)&hs, sizeof(Scal), hipHostMallocDefault));
@@ -397,16 +448,20 @@ struct Solver final : SolverBase {
   //  - ETD (withf = false): Tt, Ts = x,y-forward of tmp1 and s, from which the
   //    stage combine forms N^ (k_xa, k_zstage);
   //  - Euler (withf = true): v, f and W = x-spectra of lap f (k_xe).
-  void prefix(int mode, bool withf) {
+  // set 0: the prefix of the state (stage 1), kept in Tt, Ts; set 1: of a
+  // later stage, in Tt1, Ts1 (so the state's survives an evaluation)
+  void prefix(int mode, bool withf, int set = 0) {
+    cpx<T>* tt = set ? Tt1.p : Tt.p;
+    cpx<T>* ts = set ? Ts1.p : Ts.p;
     L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, nullptr, st); });
     XaArgs<T> a;
     a.P = P.p;
     a.Q = Q.p;
     a.v = v.p;
     a.f = f.p;
-    a.Tt = Tt.p;
+    a.Tt = tt;
     a.W = W.p;
-    a.Ts = Ts.p;
+    a.Ts = ts;
     a.part = xpart.p;
     a.tw = twx.p;
     a.npairs = npairs;
@@ -422,7 +477,7 @@ struct Solver final : SolverBase {
     if (withf) {
       spectral_lap(W.p, &fin);
     } else {
-      L("y_fwd2", [&] { ox->ypass(Tt.p, Ts.p, 2, -1, g, twx.p, &fin, st); });
+      L("y_fwd2", [&] { ox->ypass(tt, ts, 2, -1, g, twx.p, &fin, st); });
     }
     check_launch();
     prepped = false;
@@ -461,8 +516,8 @@ struct Solver final : SolverBase {
   // and the z-inverse of the next stage field into P, Q.
   void suffix(int sch, int stg, const Lut* lt, int out) {
     ZsArgs<T> z;
-    z.U = Tt.p;
-    z.U2 = Ts.p;
+    z.U = stg == 1 ? Tt.p : Tt1.p;
+    z.U2 = stg == 1 ? Ts.p : Ts1.p;
     z.Vh = Vh.p;
     z.sc = scal.p;
     z.eps = p.eps;
@@ -508,7 +563,7 @@ struct Solver final : SolverBase {
     const int out = 1 - cur;
     ensure_prefixed();
     for (int stg = 1; stg <= sch; ++stg) {
-      if (stg > 1) prefix(FIN_PLAIN, false);
+      if (stg > 1) prefix(FIN_PLAIN, false, 1);
       suffix(sch, stg, lt, out);
     }
     // after the last stage P, Q belong to uh[out]
@@ -553,13 +608,19 @@ struct Solver final : SolverBase {
   int fixed_body() {
     if (drift_mode()) {
       ensure_prefixed();
+      launch_keep(OP_KEEP, scal.p, scal_keep.p, ph, st);
       int out = eval(time_step);
       const int keep = cur;
       cur = out;
       prefix_integrals(FIN_DRIFT);   // only the drift of the targets is used
       cur = keep;
-      prepped = prefixed = prefixed_f = false;
-      ensure_prefixed();
+      // The second evaluation starts again from the kept state with the new
+      // targets.  Its prefix data (Tt, Ts) does not depend on the targets and
+      // is still there; only lambda1, lambda2 and E change.
+      launch_keep(OP_RETARGET, scal.p, scal_keep.p, ph, st);
+      prepped = false;
+      prefixed = true;
+      prefixed_f = false;
       out = eval(time_step);
       cur = out;
       prefix(FIN_DRIFT, false);
@@ -1142,7 +1203,7 @@ size_t solver_footprint(const vb_params& p) {
   const bool mixed = p.precision == VB_FP32_MIXED;
   const size_t cs = mixed ? 8 : 16, rs = cs / 2;
   const bool euler = p.scheme == VB_EULER || p.shape_adjust;
-  const size_t work = 8 + (euler ? 1 : 0);   // P, Q, Tt, Ts, pre, Na, Nb, Vh (+ W)
+  const size_t work = 10 + (euler ? 1 : 0);   // P, Q, Tt, Ts, Tt1, Ts1, pre, Na, Nb, Vh (+ W)
   size_t bytes = 3 * nc * 16 + work * nc * cs + (euler ? 2 * nr * rs : 0) + nr * 8;
   if (mixed) bytes += 3 * nc * 16;   // FP64 query buffers
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
