@@ -683,12 +683,14 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
   // N^ = -lambda1/2 delta_k0 - lambda2 3/(2 sqrt 2) F(f) - eta F(s)
   //      - (eta/eps) |k|^2 F(tmp1),  F(f) = eps |k|^2 v^ + (F(tmp1) + kappa v^)/eps.
   // Everything but the v^ terms is formed right after the transforms, so the
-  // spectrum of s is dead before the combine's operands load:
+  // spectra of tmp1 and s are dead before the combine's operands load:
   // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
+  // Np stays FP64: near an equilibrium f^ is small and its two parts cancel.
   fft2<T, N, -1>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
   const double l1h = 0.5 * a.sc->l1;
   const double l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
   const double ie = 1.0 / a.eps;
+  cpx<double> np[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
@@ -697,7 +699,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
     double pr = at * double(x[e].x) - a.eta * double(x2[e].x);
     const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
     if (id.q == 0 && kz == 0) pr -= l1h;
-    x[e] = mk<T>(T(pr), T(pi));
+    np[e] = mk<double>(pr, pi);
   }
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
   cpx<double> ur[USES_U ? E : 1];
@@ -731,7 +733,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
       vr = double(pv[e].x);
       vi = double(pv[e].y);
     }
-    const double nr = double(x[e].x) + av * vr, ni = double(x[e].y) + av * vi;   // N^ of this stage
+    const double nr = np[e].x + av * vr, ni = np[e].y + av * vi;   // N^ of this stage
     x[e] = mk<T>(T(nr), T(ni));
     double fr, fi;   // next field
     if constexpr (SCH == 4 && STG == 1) {
