@@ -378,9 +378,37 @@ struct YArgs {
   Geo g;
   FinArgs fin;  // workgroups [0, fin_on) of blockIdx.y = 0 evaluate these integrals
   int fin_on;
+  cpx<T>* o0;   // GM: out-of-place outputs in the z kernels' group-major layout
+  cpx<T>* o1;
+  int ztb;      // GM: the z kernels' column tile
 };
 
-template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false, int V = 0>
+// Position of spectral element (z, ky, kx) in the group-major layout of the z
+// stage kernels: [z workgroup][kz][ztb], the workgroup and lane that zidx
+// gives to column (ky, kx).
+__device__ __forceinline__ long long gm_index(const Geo& g, int ztb, int z, int ky, int kx) {
+  long long grp;
+  int c;
+  if (g.aligned) {
+    if (kx == g.nx / 2) {
+      grp = ky / ztb;
+      c = ky % ztb;
+    } else {
+      grp = (long long)((g.ny + ztb - 1) / ztb) + (long long)ky * ((g.nx / 2) / ztb) + kx / ztb;
+      c = kx % ztb;
+    }
+  } else {
+    const long long q = (long long)ky * g.nh + kx;
+    grp = q / ztb;
+    c = int(q % ztb);
+  }
+  return (grp * g.nz + z) * ztb + c;
+}
+
+// GM: the transform is written to o0 / o1 in the z kernels' group-major
+// layout instead of in place (the y-forward before a stage combine, so that
+// the combine reads its two inputs as contiguous runs).
+template <typename T, int N, int S, int TB = tbw<T, N>(), bool TAB = false, int V = 0, bool GM = false>
 __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, const YArgs<T>* tab) {
   constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[N * TB];
@@ -406,8 +434,15 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
   for (int e = 0; e < E; ++e) x[e] = col[(long long)(j + TF * e) * g.pitch];
   fft<T, N, S, V>(x, j, tw, lds);
   if (id.valid) {
+    if constexpr (GM) {
+      cpx<T>* out = blockIdx.y ? a.o1 : a.o0;
+      const int z = int(id.base / g.pp), kx = int(id.base % g.pp);
 #pragma unroll
-    for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
+      for (int e = 0; e < E; ++e) out[gm_index(g, a.ztb, z, j + TF * e, kx)] = x[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
+    }
   }
 }
 
@@ -625,8 +660,8 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
 
 template <typename T>
 struct ZsArgs {
-  const cpx<T>* U;        // x,y-forward of tmp1 (normalised)
-  const cpx<T>* U2;       // x,y-forward of s (normalised)
+  const cpx<T>* U;        // x,y-forward of tmp1 (normalised), group-major
+  const cpx<T>* U2;       // x,y-forward of s (normalised), group-major
   cpx<T>* Vh;             // spectrum of the stage field (stages after the first)
                           // and of the next one, group-major, in place
   const Scal* sc;         // lambda1, lambda2 of the stage field
@@ -668,16 +703,18 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g);
   const long long plane = g.pp;
-  // pre, Na, Nb and Vh live only between the stages of one evaluation, so
-  // they are kept group-major: this workgroup's TB columns of all N planes
-  // are one contiguous run (see zs_private), which keeps the strided z
-  // traffic of a stage to the spectra that other kernels share
+  // U, U2, pre, Na, Nb and Vh are read only here, so they are kept
+  // group-major: this workgroup's TB columns of all N planes are one
+  // contiguous run (see zs_private); the strided z traffic of a stage is
+  // only the state and P, Q
   const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
+  // (lanes past the last column of a Nyquist group have no input written;
+  // they transform zeros and store nothing)
   cpx<T> x[E], x2[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    x[e] = a.U[id.q + (long long)(j + TF * e) * plane];
-    x2[e] = a.U2[id.q + (long long)(j + TF * e) * plane];
+    x[e] = id.valid ? a.U[ib0 + (long long)(j + TF * e) * TB] : mk<T>(T(0), T(0));
+    x2[e] = id.valid ? a.U2[ib0 + (long long)(j + TF * e) * TB] : mk<T>(T(0), T(0));
   }
   // Calc.cu:428-444 with nld (Calc.cu:265-297) moved to spectral space:
   // N^ = -lambda1/2 delta_k0 - lambda2 3/(2 sqrt 2) F(f) - eta F(s)

@@ -17,7 +17,7 @@ namespace vb {
 // then issues one launch per recorded position for all solves of a size
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
-  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_ZLAP, OP_ZPREP,
+  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_Y_FWD_GM, OP_ZLAP, OP_ZPREP,
   OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET
 };
 // x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
@@ -68,6 +68,9 @@ struct SizeOps {
   // fin (or nullptr): integrals evaluated by extra workgroups of the launch
   void (*ypass)(cpx<T>* b0, cpx<T>* b1, int narr, int sign, const Geo& g, const cpx<T>* tw, const FinArgs* fin,
                 hipStream_t s);
+  // forward y pass of two arrays into the z kernels' group-major layout
+  void (*ypass_gm)(const cpx<T>* b0, const cpx<T>* b1, cpx<T>* o0, cpx<T>* o1, int ztb, const Geo& g,
+                   const cpx<T>* tw, const FinArgs* fin, hipStream_t s);
   void (*zpass)(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s);
   // fin (or nullptr): integrals evaluated by one extra workgroup of the launch
   void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s);

@@ -68,6 +68,16 @@ struct Launch {
     else
       hipLaunchKernelGGL((k_ypass<T, N, +1, CB, false, CV>), grid, bl, 0, s, a, nullptr);
   }
+  // forward y pass of two arrays, written out of place in the z kernels'
+  // group-major layout (column tile ztb)
+  static void ypass_gm(const cpx<T>* b0, const cpx<T>* b1, cpx<T>* o0, cpx<T>* o1, int ztb, const Geo& g,
+                       const cpx<T>* tw, const FinArgs* fin, hipStream_t s) {
+    const int nf = fin_groups(fin);
+    const dim3 grid(unsigned(ygroups(g, CB)) + unsigned(nf), 2u), bl(CT);
+    YArgs<T> a{const_cast<cpx<T>*>(b0), const_cast<cpx<T>*>(b1), tw, g, fin ? *fin : FinArgs{}, nf, o0, o1, ztb};
+    if (g_recorder) return record_launch(&launch_tab, OP_Y_FWD_GM, a, grid, bl);
+    hipLaunchKernelGGL((k_ypass<T, N, -1, CB, false, CV, true>), grid, bl, 0, s, a, nullptr);
+  }
   static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
     if (sign < 0)
@@ -132,6 +142,9 @@ struct Launch {
       case OP_Y_INV:
         hipLaunchKernelGGL((k_ypass<T, N, +1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
         break;
+      case OP_Y_FWD_GM:
+        hipLaunchKernelGGL((k_ypass<T, N, -1, CB, true, CV, true>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);
+        break;
       case OP_ZLAP:
         hipLaunchKernelGGL((k_zlap<T, N, CB, true, CV>), gr, bl, 0, s, ZlArgs<T>{}, (const ZlArgs<T>*)tab);
         break;
@@ -162,6 +175,7 @@ struct Launch {
     o.xa = &xa;
     o.xe = &xe;
     o.ypass = &ypass;
+    o.ypass_gm = &ypass_gm;
     o.zpass = &zpass;
     o.zlap = &zlap;
     o.zprep = &zprep;

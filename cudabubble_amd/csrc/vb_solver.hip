@@ -155,8 +155,9 @@ struct Solver final : SolverBase {
   const SizeOps<double>* oxd = nullptr;
   const SizeOps<double>* ozd = nullptr;
   DBuf<cpx<double>> uh[2], S;            // state (FP64) and a transform scratch
-  DBuf<cpx<T>> P, Q, Tt, Ts, W;          // transform work buffers; Tt, Ts: the state's prefix
-  DBuf<cpx<T>> Tt1, Ts1;                 // the prefixes of stages 2..4
+  DBuf<cpx<T>> P, Q, W;                  // transform work buffers
+  DBuf<cpx<T>> Tt, Ts;                   // the state's prefix (group-major)
+  DBuf<cpx<T>> Tt1, Ts1;                 // the prefixes of stages 2..4 (group-major)
   DBuf<cpx<T>> pre, Na, Nb, Vh;          // stage spectra kept for later stages (group-major)
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
@@ -262,8 +263,8 @@ struct Solver final : SolverBase {
     }
     const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
     for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
-    for (auto* b : {&P, &Q, &Tt, &Ts, &Tt1, &Ts1}) b->alloc(nc);
-    for (auto* b : {&pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
+    for (auto* b : {&P, &Q}) b->alloc(nc);
+    for (auto* b : {&Tt, &Ts, &Tt1, &Ts1, &pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
     // W, v and f carry the Euler right-hand side only (run_static, shape
     // adjust, the residual); the ETD pipeline needs no real fields stored
     if (p.scheme == VB_EULER || p.shape_adjust) {
@@ -459,9 +460,11 @@ struct Solver final : SolverBase {
     a.Q = Q.p;
     a.v = v.p;
     a.f = f.p;
-    a.Tt = tt;
+    // ETD: tmp1 and s go to P, Q in place (each lane reads its two rows of P
+    // and Q before it writes them); Euler: tmp1 to a scratch, f to W
+    a.Tt = withf ? tt : P.p;
     a.W = W.p;
-    a.Ts = ts;
+    a.Ts = Q.p;
     a.part = xpart.p;
     a.tw = twx.p;
     a.npairs = npairs;
@@ -477,7 +480,7 @@ struct Solver final : SolverBase {
     if (withf) {
       spectral_lap(W.p, &fin);
     } else {
-      L("y_fwd2", [&] { ox->ypass(tt, ts, 2, -1, g, twx.p, &fin, st); });
+      L("y_fwd2", [&] { ox->ypass_gm(P.p, Q.p, tt, ts, oz->ytile, g, twx.p, &fin, st); });
     }
     check_launch();
     prepped = false;
