@@ -691,10 +691,13 @@ struct ZsArgs {
 // All stages load their operands after the forward transforms, and ask for
 // no minimum occupancy: a cap of three or four waves per SIMD spills heavily
 // and runs slower.
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false>
-__global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
+// Plan variant of the stage combines (see Plan).
+template <typename T, int N> constexpr int zsplan() { return 0; }
+
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>()>
+__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int E = Plan<N>::E, TF = N / E;
+  constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
@@ -723,7 +726,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
   // spectra of tmp1 and s are dead before the combine's operands load:
   // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
   // Np stays FP64: near an equilibrium f^ is small and its two parts cancel.
-  fft2<T, N, -1>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
+  fft2<T, N, -1, V>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
   const double l1h = 0.5 * a.sc->l1;
   const double l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
   const double ie = 1.0 / a.eps;
@@ -823,7 +826,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zstage(ZsArgs<T> a0, const
     next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
-  next_field_tail<T, N, TB, 0>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB, V>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
 // widening copy of a spectrum into the FP64 state

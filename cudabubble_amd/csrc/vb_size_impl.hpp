@@ -101,7 +101,7 @@ struct Launch {
   }
   template <int SCH, int STG, bool TAB>
   static void zs_launch(dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB>), gr, dim3(YT), 0, s, a, tab);
+    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB>), gr, dim3(TB * TFof<N, zsplan<T, N>()>()), 0, s, a, tab);
   }
   template <bool TAB>
   static void zs_dispatch(int sch, int stg, dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
@@ -120,7 +120,9 @@ struct Launch {
   }
   static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
     const dim3 gr(zgrid(a.g));
-    if (g_recorder) return record_launch(&launch_tab, (sch == 4 ? OP_ZS41 : OP_ZS21) + (stg - 1), a, gr, dim3(YT));
+    if (g_recorder)
+      return record_launch(&launch_tab, (sch == 4 ? OP_ZS41 : OP_ZS21) + (stg - 1), a, gr,
+                           dim3(TB * TFof<N, zsplan<T, N>()>()));
     zs_dispatch<false>(sch, stg, gr, a, nullptr, s);
   }
 
