@@ -624,7 +624,7 @@ __device__ __forceinline__ void next_field_tail(cpx<T>* x, cpx<T>* y, double acc
 
 template <typename T>
 struct ZprArgs {
-  const cpx<double>* vh;
+  const cpx<double>* vh;   // state spectrum, group-major
   cpx<T>* P;
   cpx<T>* Q;
   double* part;
@@ -646,10 +646,11 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const ZIdx id = zidx<TB>(g);
-  const long long plane = g.pp;
+  // the state spectrum is group-major (see k_zstage)
+  const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
   cpx<double> u[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) u[e] = vh[id.q + (long long)(j + TF * e) * plane];
+  for (int e = 0; e < E; ++e) u[e] = id.valid ? vh[ib0 + (long long)(j + TF * e) * TB] : mk<double>(0.0, 0.0);
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = u[0].x;
   cpx<T> x[E], y[E];
   double acc = 0.0;
@@ -666,11 +667,11 @@ struct ZsArgs {
                           // and of the next one, group-major, in place
   const Scal* sc;         // lambda1, lambda2 of the stage field
   double eps, kappa, eta;
-  const cpx<double>* uh;  // state spectrum u^
+  const cpx<double>* uh;  // state spectrum u^ (group-major)
   cpx<T>* pre;            // ETDRK4: e1 a^ - q N^u (stage 1 -> 3)
   cpx<T>* Na;             // ETDRK4: N^a (stage 2 -> 4); ETDRK2: N^u (stage 1 -> 2)
   cpx<T>* Nb;             // ETDRK4: N^b (stage 3 -> 4)
-  cpx<double>* uout;      // new state spectrum (last stage)
+  cpx<double>* uout;      // new state spectrum (last stage, group-major)
   cpx<T>* P;
   cpx<T>* Q;
   double* part;
@@ -705,11 +706,9 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g);
-  const long long plane = g.pp;
-  // U, U2, pre, Na, Nb and Vh are read only here, so they are kept
-  // group-major: this workgroup's TB columns of all N planes are one
-  // contiguous run (see zs_private); the strided z traffic of a stage is
-  // only the state and P, Q
+  // The state and U, U2, pre, Na, Nb, Vh are kept group-major: this
+  // workgroup's TB columns of all N planes are one contiguous run (see
+  // zs_private); the strided z traffic of a stage is only P, Q
   const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
   // (lanes past the last column of a Nyquist group have no input written;
   // they transform zeros and store nothing)
@@ -748,9 +747,8 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   cpx<T> pv[STG > 1 ? E : 1];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const long long i = id.q + (long long)(j + TF * e) * plane;
     const long long ib = ib0 + (long long)(j + TF * e) * TB;
-    if constexpr (USES_U) ur[e] = a.uh[i];
+    if constexpr (USES_U) ur[e] = id.valid ? a.uh[ib] : mk<double>(0.0, 0.0);
     if constexpr (SCH == 4 && STG == 3) pa[e] = a.pre[ib];
     if constexpr (SCH == 4 && STG == 4) { pa[e] = a.Na[ib]; pb[e] = a.Nb[ib]; }
     if constexpr (SCH == 2 && STG == 2) pa[e] = a.Na[ib];
@@ -761,7 +759,6 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
-    const long long i = id.q + (long long)kz * plane;
     const long long ib = ib0 + (long long)kz * TB;
     const int m = mindex(g, id.kx, id.ky, kz);
     const double av = -l2c * (a.eps * double(m) * g.kfac2 + a.kappa * ie);
@@ -801,7 +798,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
       // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
       fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
       fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
-      if (id.valid) a.uout[i] = mk<double>(fr, fi);
+      if (id.valid) a.uout[ib] = mk<double>(fr, fi);
     } else if constexpr (SCH == 2 && STG == 1) {
       const double2 c = a.c1[m];                      // e^z, q1
       const cpx<double> u = ur[e];
@@ -817,7 +814,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
       const double ar = c.x * u.x + c.y * double(nu.x), ai = c.x * u.y + c.y * double(nu.y);
       fr = ar + c2 * (nr - double(nu.x));
       fi = ai + c2 * (ni - double(nu.y));
-      if (id.valid) a.uout[i] = mk<double>(fr, fi);
+      if (id.valid) a.uout[ib] = mk<double>(fr, fi);
     }
     if constexpr (STG < SCH) {
       if (id.valid) a.Vh[ib] = mk<T>(T(fr), T(fi));
@@ -829,11 +826,21 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   next_field_tail<T, N, TB, V>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
-// widening copy of a spectrum into the FP64 state
-template <typename T>
-__global__ void k_widen(const cpx<T>* __restrict__ in, cpx<double>* __restrict__ out, long long n) {
+// Copy of a half spectrum between layouts, widened to FP64: tb_in / tb_out =
+// 0 for the row layout [z][ky][pitch], or the column tile of the z kernels'
+// group-major layout (gm_index).  One thread per mode.
+template <typename TI>
+__global__ void k_relayout(const cpx<TI>* __restrict__ in, int tb_in, cpx<double>* __restrict__ out, int tb_out,
+                           Geo g) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) out[i] = mk<double>(double(in[i].x), double(in[i].y));
+  const long long nm = (long long)g.nz * g.ny * (g.nx / 2 + 1);
+  if (i >= nm) return;
+  const int kx = int(i % (g.nx / 2 + 1));
+  const int ky = int((i / (g.nx / 2 + 1)) % g.ny);
+  const int z = int(i / ((long long)(g.nx / 2 + 1) * g.ny));
+  const long long rw = (long long)z * g.pp + (long long)ky * g.pitch + kx;
+  const cpx<TI> v = in[tb_in ? gm_index(g, tb_in, z, ky, kx) : rw];
+  out[tb_out ? gm_index(g, tb_out, z, ky, kx) : rw] = mk<double>(double(v.x), double(v.y));
 }
 
 }  // namespace vb

@@ -154,7 +154,7 @@ struct Solver final : SolverBase {
   // FP64 operations (state transforms, FP64 queries in mixed mode)
   const SizeOps<double>* oxd = nullptr;
   const SizeOps<double>* ozd = nullptr;
-  DBuf<cpx<double>> uh[2], S;            // state (FP64) and a transform scratch
+  DBuf<cpx<double>> uh[2], S;            // state (FP64, group-major) and a transform scratch
   DBuf<cpx<T>> P, Q, W;                  // transform work buffers
   DBuf<cpx<T>> Tt, Ts;                   // the state's prefix (group-major)
   DBuf<cpx<T>> Tt1, Ts1;                 // the prefixes of stages 2..4 (group-major)
@@ -262,7 +262,10 @@ struct Solver final : SolverBase {
       own_stream = true;
     }
     const size_t nc = (size_t)g.ncp, nr = (size_t)g.nr;
-    for (auto* b : {&uh[0], &uh[1], &S}) b->alloc(nc);
+    for (auto* b : {&uh[0], &uh[1]}) b->alloc((size_t)zs_private(g, oz->ytile));
+    // S holds the state in the row layout (phi in and out) or regrouped for
+    // the FP64 z kernels of the mixed-mode integrals
+    S.alloc((size_t)std::max(zs_private(g, oz->ytile), zs_private(g, ozd->ytile)));
     for (auto* b : {&P, &Q}) b->alloc(nc);
     for (auto* b : {&Tt, &Ts, &Tt1, &Ts1, &pre, &Na, &Nb, &Vh}) b->alloc((size_t)zs_private(g, oz->ytile));
     // W, v and f carry the Euler right-hand side only (run_static, shape
@@ -428,6 +431,12 @@ struct Solver final : SolverBase {
   hipStream_t stream() override { return st; }
 
   // ---- pipeline pieces ------------------------------------------------------
+  // copy of a spectrum between the row layout (tile 0) and a group-major one
+  template <typename TI>
+  void relayout(const cpx<TI>* in, int tb_in, cpx<double>* out, int tb_out) {
+    const long long nm = (long long)g.nz * g.ny * (g.nx / 2 + 1);
+    hipLaunchKernelGGL(k_relayout<TI>, dim3(unsigned((nm + 255) / 256)), dim3(256), 0, st, in, tb_in, out, tb_out, g);
+  }
   void check_launch() { VB_HIP(hipGetLastError()); }
 
   void prep(const cpx<double>* vh) {
@@ -596,7 +605,7 @@ struct Solver final : SolverBase {
     L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, nullptr, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
     L("widen", [&] {
-      hipLaunchKernelGGL(k_widen<T>, dim3(unsigned((g.ncp + 255) / 256)), dim3(256), 0, st, W.p, uh[out].p, g.ncp);
+      relayout(W.p, 0, uh[out].p, oz->ytile);
     });
     check_launch();
     prefixed_f = false;   // W was consumed
@@ -1069,7 +1078,9 @@ struct Solver final : SolverBase {
     const int nzb64 = int(zgroups(g, ozd->ytile));
     if (xpart64.n < (size_t)nxb64 * 3) xpart64.alloc((size_t)nxb64 * 3);
     if (zpart64.n < (size_t)nzb64) zpart64.alloc((size_t)nzb64);
-    L("zprep64", [&] { ozd->zprep(uh[cur].p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
+    // the FP64 z kernels have their own column tile: regroup the state for them
+    L("regroup", [&] { relayout(uh[cur].p, oz->ytile, S.p, ozd->ytile); });
+    L("zprep64", [&] { ozd->zprep(S.p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
     L("y_inv2_64", [&] { oxd->ypass(P64.p, Q64.p, 2, +1, g, twxd.p, nullptr, st); });
     XaArgs<double> a;
     a.P = P64.p;
@@ -1133,7 +1144,7 @@ struct Solver final : SolverBase {
   }
 
   void get_phi(double* host) override {
-    VB_HIP(hipMemcpyAsync(S.p, uh[cur].p, sizeof(cpx<double>) * (size_t)g.ncp, hipMemcpyDeviceToDevice, st));
+    L("to_rows", [&] { relayout(uh[cur].p, oz->ytile, S.p, 0); });
     L("z_inv", [&] { ozd->zpass(S.p, +1, g, twzd.p, st); });
     L("y_inv", [&] { oxd->ypass(S.p, nullptr, 1, +1, g, twxd.p, nullptr, st); });
     L("x_c2r", [&] { oxd->xc2r_d(S.p, phid.p, npairs, twxd.p, g, st); });
@@ -1146,9 +1157,10 @@ struct Solver final : SolverBase {
     cur_E_known = false;
     iter_open = false;
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
-    L("x_r2c", [&] { oxd->xr2c_d(phid.p, uh[cur].p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
-    L("y_fwd", [&] { oxd->ypass(uh[cur].p, nullptr, 1, -1, g, twxd.p, nullptr, st); });
-    L("z_fwd", [&] { ozd->zpass(uh[cur].p, -1, g, twzd.p, st); });
+    L("x_r2c", [&] { oxd->xr2c_d(phid.p, S.p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
+    L("y_fwd", [&] { oxd->ypass(S.p, nullptr, 1, -1, g, twxd.p, nullptr, st); });
+    L("z_fwd", [&] { ozd->zpass(S.p, -1, g, twzd.p, st); });
+    L("to_groups", [&] { relayout(S.p, 0, uh[cur].p, oz->ytile); });
     check_launch();
     VB_HIP(hipStreamSynchronize(st));
     prepped = prefixed = prefixed_f = false;
