@@ -42,7 +42,7 @@ def alg_bytes(name, n, prec_bytes):
         "y_inv2": 4 * C, "y_fwd2": 4 * C, "y_fwd": 2 * C, "y_inv": 2 * C, "z_lap": 2 * C, "z_lap_fin": 2 * C,
         "z_fwd": 2 * C, "z_inv": 2 * C,
         # xa: P, Q in; spectra of tmp1 and s out (Euler: tmp1, f and the real v, f)
-        "xa": 4 * C, "xa_f": 4 * C + 2 * R, "xa_sums": 2 * C,
+        "xa": 4 * C, "xa_f": 4 * C + 2 * R, "xa_sums": C,
         "zprep": modes * (16 + 2 * cs),
         # stage combines: spectra of tmp1 and s in, the stage field's spectrum
         # (state or Vh), stage/state spectra; next-stage field (Vh), P, Q out

@@ -175,9 +175,7 @@ struct XaArgs {
 // (F(tmp1) + kappa v^)/eps.  This kernel writes the x-forward of tmp1 and s;
 // the stage combine forms N^ from their spectra, so lap(tmp1) never goes back
 // to real space.
-// SUMS: the partial sums only (the integrals of a field whose stage data is
-// not needed, as in the first half of the penalty-drift double step).
-template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false, bool SUMS = false>
+template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false>
 __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
@@ -231,9 +229,7 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
       }
     }
   }
-  if constexpr (SUMS) {
-    // no transform out
-  } else if constexpr (WITHS) {   // one transform after the other: fewer live registers
+  if constexpr (WITHS) {   // one transform after the other: fewer live registers
     {
       cpx<T> z[E];
 #pragma unroll
@@ -259,6 +255,38 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
     for (int e = 0; e < E; ++e) z[e] = mk<T>(ff[0][e], ff[1][e]);
     fft<T, N, -1, XP>(z, j, a.tw, lds);
     unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+  }
+  block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
+}
+
+// The partial sums of v and (v^2-1)^2 from P alone, for the targets' drift in
+// the first half of the penalty-drift double step (it needs V and A only, not
+// E_b; the sum of f^2 is left zero).  Two rows of P travel as one transform.
+template <typename T, int N, bool TAB = false>
+__global__ void __launch_bounds__(XWG) k_xsums(XaArgs<T> a0, const XaArgs<T>* tab) {
+  const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>();
+  __shared__ cpx<T> sm[G * LL];
+  __shared__ double red[3 * (XWG / 64)];
+  const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
+  RowLds<T> lds{sm + lane * LL};
+  long long pair = (long long)blockIdx.x * G + lane;
+  const bool valid = pair < a.npairs;
+  if (!valid) pair = a.npairs - 1;
+  const long long r0 = 2 * pair;
+  cpx<T> x[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(r0), a.P + a.rl.off(r0 + 1), j + TF * e);
+  fft<T, N, +1, XP>(x, j, a.tw, lds);
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double v = double(h ? x[e].y : x[e].x);
+      const double w = v * v - 1.0;
+      if (valid) { acc[0] += v; acc[1] += w * w; }
+    }
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }

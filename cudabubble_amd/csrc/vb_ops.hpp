@@ -21,7 +21,7 @@ enum PipeOp {
   OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET
 };
 // x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
-// the partial sums only
+// the partial sums of v and (v^2-1)^2 only (from P alone: k_xsums)
 enum XaMode { XA_PLAIN = 0, XA_WITHF = 1, XA_WITHS = 2, XA_SUMS = 3 };
 struct RecLaunch {
   // issues the table form of the recorded kernel over `lanes` solves

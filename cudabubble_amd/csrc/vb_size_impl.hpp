@@ -50,7 +50,7 @@ struct Launch {
     else if (mode == XA_WITHS)
       hipLaunchKernelGGL((k_xa<T, N, false, false, true>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_SUMS)
-      hipLaunchKernelGGL((k_xa<T, N, false, false, false, true>), gr, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_xsums<T, N>), gr, bl, 0, s, a, nullptr);
     else
       hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
   }
@@ -136,7 +136,7 @@ struct Launch {
         hipLaunchKernelGGL((k_xa<T, N, false, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
       case OP_XA_SUMS:
-        hipLaunchKernelGGL((k_xa<T, N, false, true, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        hipLaunchKernelGGL((k_xsums<T, N, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
       case OP_Y_FWD:
         hipLaunchKernelGGL((k_ypass<T, N, -1, CB, true, CV>), gr, bl, 0, s, YArgs<T>{}, (const YArgs<T>*)tab);

@@ -497,10 +497,11 @@ struct Solver final : SolverBase {
     prefixed_f = withf;
   }
 
-  // The integrals of the prepared stage field alone (no stage data): y-inverse,
-  // the x kernel's partial sums, and the finalize kernel.
+  // V and A of the prepared stage field alone (no stage data, and no E_b):
+  // y-inverse of P, the partial sums of v and (v^2-1)^2, and the finalize
+  // kernel.  Used where only the targets' drift is needed.
   void prefix_integrals(int mode) {
-    L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, nullptr, st); });
+    L("y_inv", [&] { ox->ypass(P.p, nullptr, 1, +1, g, twx.p, nullptr, st); });
     XaArgs<T> a{};
     a.P = P.p;
     a.Q = Q.p;

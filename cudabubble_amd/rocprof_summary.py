@@ -42,12 +42,12 @@ def solver_name(mangled):
         return None
     prec = "fp64" if args[0] == "double" else "fp32"
     a = args[1:] + [None] * 6
-    if kind == "xa":   # <T, N, WITHF, TAB, WITHS, SUMS>
+    if kind == "xa":   # <T, N, WITHF, TAB, WITHS>
         if a[1] == "true":
             return "xa_f", prec
-        if a[4] == "true":
-            return "xa_sums", prec
         return ("xa" if a[3] == "true" else "xa_plain"), prec
+    if kind == "xsums":
+        return "xa_sums", prec
     if kind == "ypass":
         return ("y_fwd" if a[1] == "-1" else "y_inv"), prec
     if kind == "zpass":
