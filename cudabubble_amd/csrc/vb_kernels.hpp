@@ -38,17 +38,21 @@ template <typename T, bool WITHS = false> constexpr int xwaves() { return (sizeo
 // 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
 template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && N == 256) ? 1 : 0; }
 
-// Column tile width of the y / z kernels (columns that are contiguous in
-// memory): enough columns for >= 128-byte segments and >= 256 threads.
-template <typename T, int N> constexpr int tbw() {
+// Column tile width of the y passes and the z Laplacian (columns that are
+// contiguous in memory): enough columns for >= 128-byte segments and >= 256
+// threads.
+template <typename T, int N> constexpr int colw() {
   constexpr int seg = 128 / int(sizeof(cpx<T>));
   constexpr int occ = 256 / TFof<N>();
   return seg > occ ? seg : occ;
 }
 
-// Column tile width of the y passes and the z Laplacian, which keep one
-// column set in LDS (the stage combines keep two and stay at tbw).
-template <typename T, int N> constexpr int colw() { return tbw<T, N>(); }
+// Column tile of the z stage kernels (stage combines, z-prep, plain z
+// passes): the column width, except FP32 at 256, where 8 columns (256-thread
+// workgroups, two or three per CU) measured 6-14 % faster per stage than 16:
+// their inputs are group-major, so only their P, Q outputs are written in
+// 64-byte pieces.
+template <typename T, int N> constexpr int tbw() { return (sizeof(T) == 4 && N == 256) ? 8 : colw<T, N>(); }
 
 // Z_p = A_ext(p) + i * B_ext(p): Hermitian extension along x of two half
 // spectra.  The imaginary parts at kx = 0 and kx = N/2 are dropped, which is
