@@ -5,17 +5,17 @@
 //
 //   z-kernel  : v^ -> z-inverse of v^ and of -|k|^2 v^   (+ Parseval |grad v|^2)
 //   y-kernel  : y-inverse of both
-//   xa-kernel : x-inverse (two-for-one) -> v, lap v; f, tmp1, s, sums of v,
-//               (v^2-1)^2, f^2; x-forward of tmp1 and s
+//   xrow      : x-inverse of v^ + i lap^ v -> v, lap v; f, tmp1, s, sums of
+//               v, (v^2-1)^2, f^2; x-forward of tmp1 + i s
 //   y-kernel  : y-forward of both; extra workgroups of the launch evaluate
 //               V, A, E_b, lambda1, lambda2, E from the block partials
 //               (finalize_body)
-//   zstage    : z-forward of both -> N^ in spectral space (k_xa explains
+//   zstage    : z-forward of both -> N^ in spectral space (k_xrow explains
 //               why that is exact), the ETD combine with the state, and the
 //               first kernel of the next stage (z-inverse of the next field)
 //
-// The Euler paths keep a real-space right-hand side: xa also stores v and f
-// and transforms f, and y, z(lap), y kernels take lap f back to x-spectra.
+// The Euler paths keep a real-space right-hand side: their x kernel (k_xa)
+// stores v and f and transforms f, and y, z(lap), y kernels take lap f back to x-spectra.
 #pragma once
 #include "vb_device.hpp"
 
@@ -29,10 +29,8 @@ template <typename T, int N> constexpr int xlanes() { return XWG / TFof<N, xplan
 
 // Minimum waves per SIMD requested for the fused x kernels: in FP32 four
 // (the register budget that allows it costs at most a few spilled words and
-// gains more in latency hiding than it loses); FP64, and the FP32 x kernel of
-// the ETD stages (two forward transforms out, about 200 registers; a cap of
-// three or four spills and ran 33 % slower), keep the full budget.
-template <typename T, bool WITHS = false> constexpr int xwaves() { return (sizeof(T) == 4 && !WITHS) ? 4 : 1; }
+// gains more in latency hiding than it loses); FP64 keeps the full budget.
+template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
 
 // Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256:
 // 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
@@ -78,7 +76,7 @@ struct Rows {
 // spectra of a and b, times s.  Uses the lane's LDS; ends with a barrier.
 template <typename T, int N, int V = 0, class Lds>
 __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, cpx<T>* __restrict__ outA,
-                                             cpx<T>* __restrict__ outB, T s, bool valid) {
+                                             cpx<T>* __restrict__ outB, T s, bool valid, T sb = T(1)) {
   constexpr int E = Plan<N, V>::E, TF = N / E, NH = N / 2 + 1;
 #pragma unroll
   for (int e = 0; e < E; ++e) lds(j + TF * e) = x[e];
@@ -89,10 +87,10 @@ __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, c
     if (k < NH && valid) {
       const cpx<T> zk = lds(k);
       const cpx<T> zm = lds(k == 0 ? 0 : N - k);
-      const T h = T(0.5) * s;
+      const T h = T(0.5) * s, hb = h * sb;
       // a_k = (Z_k + conj Z_{N-k}) / 2,  b_k = (Z_k - conj Z_{N-k}) / (2i)
       outA[k] = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
-      outB[k] = mk<T>((zk.y + zm.y) * h, (zm.x - zk.x) * h);
+      outB[k] = mk<T>((zk.y + zm.y) * hb, (zm.x - zk.x) * hb);
     }
   }
   __syncthreads();
@@ -161,7 +159,7 @@ struct XaArgs {
   T* f;
   cpx<T>* Tt;        // out: x-forward of tmp1, times inv_n
   cpx<T>* W;         // out (Euler only): x-forward of f, times inv_n
-  cpx<T>* Ts;        // out (WITHS): x-forward of s, times inv_n
+  cpx<T>* Ts;        // out (k_xrow): x-forward of s, times inv_n
   double* part;      // [gridDim.x][3]
   const cpx<T>* tw;
   long long npairs;
@@ -172,15 +170,11 @@ struct XaArgs {
 // Every pipeline kernel takes its argument struct by value, or (TAB, a
 // batched launch over the solves of a size group) from a per-solve table in
 // device memory indexed by blockIdx.z.
-// WITHS: the spectral form of the nonlinear term.  nld = -lap(tmp1)/eps + s
-// with s = D tmp1/eps^3 + (3v^2 - D - 1) f/eps^2 (Calc.cu:278-282), and the
-// forward transform is linear, so N^ = -lambda1/2 delta_k0 - lambda2 c F(f)
-// - eta F(s) - (eta/eps) |k|^2 F(tmp1), with F(f) = eps |k|^2 v^ +
-// (F(tmp1) + kappa v^)/eps.  This kernel writes the x-forward of tmp1 and s;
-// the stage combine forms N^ from their spectra, so lap(tmp1) never goes back
-// to real space.
-template <typename T, int N, bool WITHF, bool TAB = false, bool WITHS = false>
-__global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
+// k_xa: the x kernel of the Euler paths and of the reference-dataflow
+// evaluation (tmp1 out, and with WITHF v, f and the x-forward of f), two rows
+// per lane.
+template <typename T, int N, bool WITHF, bool TAB = false>
+__global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, G = xlanes<T, N>(), LL = row_lds_len<N>(), NH = N / 2 + 1;
   __shared__ cpx<T> sm[2 * G * LL];
@@ -200,7 +194,6 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
   double acc[3] = {0.0, 0.0, 0.0};
   T tm[2][E];
   T ff[WITHF ? 2 : 1][E];
-  T ss[WITHS ? 2 : 1][E];
   {
     cpx<T> x[2][E];
 #pragma unroll
@@ -229,24 +222,10 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
         }
         tm[h][e] = T(t1);
         if constexpr (WITHF) ff[h][e] = T(f);
-        if constexpr (WITHS) ss[h][e] = T(D * t1 * ie3 + (3 * v * v - D - 1) * ie2 * f);   // Calc.cu:282
       }
     }
   }
-  if constexpr (WITHS) {   // one transform after the other: fewer live registers
-    {
-      cpx<T> z[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
-      fft<T, N, -1, XP>(z, j, a.tw, lds);
-      unpack_store<T, N, XP>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
-    }
-    cpx<T> z[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) z[e] = mk<T>(ss[0][e], ss[1][e]);
-    fft<T, N, -1, XP>(z, j, a.tw, lds2);
-    unpack_store<T, N, XP>(z, j, lds2, a.Ts + a.rl.off(r0), a.Ts + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
-  } else {
+  {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
@@ -261,6 +240,58 @@ __global__ void __launch_bounds__(XWG, (xwaves<T, WITHS>())) k_xa(XaArgs<T> a0, 
     unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
+}
+
+// The x kernel of an ETD stage, one row per lane: the x-inverse of P + i Q of
+// the row gives v and lap v; f, tmp1 and s follow pointwise; the x-forward of
+// tmp1 + i s gives both half spectra, written over the row's P and Q.  One
+// transform each way per lane: the two-row form needs about 200 registers and
+// two waves per SIMD, this one about 120 and four, and ran 9 % faster at
+// 256^3.  2 * XWG threads cover as many rows as the other x kernels'
+// workgroups, so the partial sums line up.
+//
+// The spectral form of the nonlinear term: nld = -lap(tmp1)/eps + s with
+// s = D tmp1/eps^3 + (3v^2 - D - 1) f/eps^2 (Calc.cu:278-282), and the
+// forward transform is linear, so N^ = -lambda1/2 delta_k0 - lambda2 c F(f)
+// - eta F(s) - (eta/eps) |k|^2 F(tmp1), with F(f) = eps |k|^2 v^ +
+// (F(tmp1) + kappa v^)/eps.  The stage combine forms N^ from the spectra of
+// tmp1 and s, so lap(tmp1) never goes back to real space.  s is O(1/eps^3)
+// where tmp1 is O(1), and the rounding of a shared transform scales with the
+// larger part: s goes in as eps^3 s, of tmp1's size, and its spectrum is
+// scaled back at the store.
+template <typename T, int N, bool TAB = false>
+__global__ void __launch_bounds__(2 * XWG) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
+  const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(),
+                LL = row_lds_len<N>();
+  __shared__ cpx<T> sm[RL * LL];
+  __shared__ double red[3 * (2 * XWG / 64)];
+  const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
+  RowLds<T> lds{sm + lane * LL};
+  const long long nrows = 2 * a.npairs;
+  long long row = (long long)blockIdx.x * RL + lane;
+  const bool valid = row < nrows;
+  if (!valid) row = nrows - 1;
+  const double eps = a.eps, kappa = a.kappa, ieps = 1.0 / eps;
+  const double D = a.D, e3 = eps * eps * eps;
+  cpx<T> x[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(row), a.Q + a.rl.off(row), j + TF * e);
+  fft<T, N, +1, XP>(x, j, a.tw, lds);   // v + i lap v
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const double v = double(x[e].x), lv = double(x[e].y);
+    const double w = v * v - 1.0;
+    const double f = -lv * eps + w * v * ieps;                          // f_kernel, Calc.cu:160
+    const double t1 = v * (v * v - 1.0 - kappa);                        // nld_kernel1, Calc.cu:270
+    const double sv = D * t1 + (3 * v * v - D - 1) * eps * f;        // eps^3 s, Calc.cu:282
+    if (valid) { acc[0] += v; acc[1] += w * w; acc[2] += f * f; }
+    x[e] = mk<T>(T(t1), T(sv));
+  }
+  fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
+  unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
+  block_sum_store<3, 2 * XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
 // The partial sums of v and (v^2-1)^2 from P alone, for the targets' drift in
@@ -716,7 +747,7 @@ struct ZsArgs {
 
 // SCH = 4 (ETDRK4, belastic.cpp:519-552 / Calc.cu:488-577) or 2 (ETDRK2,
 // belastic.cpp:491-517 / Calc.cu:446-486); STG = stage whose N^ this is.
-// The stage's N^ is formed here from the spectra of tmp1 and s (see k_xa),
+// The stage's N^ is formed here from the spectra of tmp1 and s (see k_xrow),
 // the stage field's spectrum v^ (the state u^ in the first stage, Vh after
 // it) and lambda1, lambda2; the stage also stores the next stage field's
 // spectrum in Vh.  The N^ of a stage is stored in precision T for a later

@@ -40,7 +40,8 @@ struct Launch {
     hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
   static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
-    const dim3 gr(xgrid(a.npairs)), bl(XWG);
+    // k_xrow: one row per lane, twice the threads over the same rows
+    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? 2 * XWG : XWG);
     if (g_recorder) {
       const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
       return record_launch(&launch_tab, op, a, gr, bl);
@@ -48,7 +49,7 @@ struct Launch {
     if (mode == XA_WITHF)
       hipLaunchKernelGGL((k_xa<T, N, true>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_WITHS)
-      hipLaunchKernelGGL((k_xa<T, N, false, false, true>), gr, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_xrow<T, N>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_SUMS)
       hipLaunchKernelGGL((k_xsums<T, N>), gr, bl, 0, s, a, nullptr);
     else
@@ -133,7 +134,7 @@ struct Launch {
       case OP_XA: hipLaunchKernelGGL((k_xa<T, N, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_S:
-        hipLaunchKernelGGL((k_xa<T, N, false, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        hipLaunchKernelGGL((k_xrow<T, N, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
       case OP_XA_SUMS:
         hipLaunchKernelGGL((k_xsums<T, N, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);

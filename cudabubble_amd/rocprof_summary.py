@@ -42,10 +42,10 @@ def solver_name(mangled):
         return None
     prec = "fp64" if args[0] == "double" else "fp32"
     a = args[1:] + [None] * 6
-    if kind == "xa":   # <T, N, WITHF, TAB, WITHS>
-        if a[1] == "true":
-            return "xa_f", prec
-        return ("xa" if a[3] == "true" else "xa_plain"), prec
+    if kind == "xa":   # <T, N, WITHF, TAB>
+        return ("xa_f" if a[1] == "true" else "xa_plain"), prec
+    if kind == "xrow":   # the x kernel of the ETD stages
+        return "xa", prec
     if kind == "xsums":
         return "xa_sums", prec
     if kind == "ypass":
