@@ -263,13 +263,19 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
+// Plan of k_xrow: in FP32 at 256 the 16 x 16 plan (one LDS exchange per
+// transform instead of two; 204 registers, two waves per SIMD) ran 71 -> 65 us
+// against 8 x 8 x 4 (116 registers, four waves).
+template <typename T, int N> constexpr int xrplan() { return (sizeof(T) == 4 && N == 256) ? 1 : xplan<T, N>(); }
+// threads of a k_xrow workgroup: as many rows as the other x kernels'
+// workgroups (two per row pair), so the partial sums line up
+template <typename T, int N> constexpr int xrthreads() { return 2 * xlanes<T, N>() * TFof<N, xrplan<T, N>()>(); }
+
 // The x kernel of an ETD stage, one row per lane: the x-inverse of P + i Q of
 // the row gives v and lap v; f, tmp1 and s follow pointwise; the x-forward of
 // tmp1 + i s gives both half spectra, written over the row's P and Q.  One
-// transform each way per lane: the two-row form needs about 200 registers and
-// two waves per SIMD, this one about 120 and four, and ran 9 % faster at
-// 256^3.  2 * XWG threads cover as many rows as the other x kernels'
-// workgroups, so the partial sums line up.
+// transform each way per lane: the two-row form needs about 200 registers
+// with the 8 x 8 x 4 plan and ran 9 % slower at 256^3.
 //
 // The spectral form of the nonlinear term: nld = -lap(tmp1)/eps + s with
 // s = D tmp1/eps^3 + (3v^2 - D - 1) f/eps^2 (Calc.cu:278-282), and the
@@ -281,12 +287,13 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
 // larger part: s goes in as eps^3 s, of tmp1's size, and its spectrum is
 // scaled back at the store.
 template <typename T, int N, bool TAB = false>
-__global__ void __launch_bounds__(2 * XWG) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
+__global__ void __launch_bounds__((xrthreads<T, N>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int XP = xplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(),
-                LL = row_lds_len<N>();
+  constexpr int XP = xrplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(),
+                LL = row_lds_len<N>(), NT = xrthreads<T, N>();
+  static_assert(NT % 64 == 0, "whole waves");
   __shared__ cpx<T> sm[RL * LL];
-  __shared__ double red[3 * (2 * XWG / 64)];
+  __shared__ double red[3 * (NT / 64)];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
   const long long nrows = 2 * a.npairs;
@@ -312,7 +319,7 @@ __global__ void __launch_bounds__(2 * XWG) k_xrow(XaArgs<T> a0, const XaArgs<T>*
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
   unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
-  block_sum_store<3, 2 * XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
+  block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
 // The partial sums of v and (v^2-1)^2 from P alone, for the targets' drift in

@@ -40,8 +40,8 @@ struct Launch {
     hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
   static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
-    // k_xrow: one row per lane, twice the threads over the same rows
-    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? 2 * XWG : XWG);
+    // k_xrow: one row per lane, over the same rows per workgroup
+    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? xrthreads<T, N>() : XWG);
     if (g_recorder) {
       const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
       return record_launch(&launch_tab, op, a, gr, bl);
