@@ -12,7 +12,9 @@ FFTs and pointwise stages, FP64 coefficient tables, reductions and penalty
 logic).  The FP64 rate, the FP32-vs-FP64 error after the same steps, the
 per-step latency percentiles, the roofline of the dominant kernel and a CPU
 baseline (the FP64 oracle on the host cores, a bounded sample) are reported
-in the same JSON line.
+in the same JSON line.  Before the FP32-mixed solves of a grid size are built,
+vb_autotune times the kernel schedules compiled for that size and selects the
+fastest (--no-tune keeps schedule 0); the line reports its timings and choice.
 """
 import argparse
 import json
@@ -319,6 +321,8 @@ def main():
                     help="steps of the 48-solve batched form (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="arrival window of the serving form at 20 requests/s (0: skip it)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep schedule 0 instead of autotuning the kernel schedules of each grid size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -333,8 +337,17 @@ def main():
     import cudabubble_amd as vb
     vb.lib()
     n = args.n
+    tuned = {}
+
+    def tune(size):
+        # vb_autotune: time every compiled kernel schedule of this size and keep the fastest
+        if not args.no_tune and "%d" % size not in tuned:
+            r = vb.autotune(size, vb.FP32_MIXED, steps=4)
+            tuned["%d" % size] = {"chosen": r["chosen"], "ms_per_step": [round(x, 4) for x in r["ms_per_step"]],
+                                  "schedules": r["schedules"]}
 
     # FP32-mixed (headline)
+    tune(n)
     p32, phi, sp = config_params(vb, n, vb.FP32_MIXED, vb.LUT_ACCURATE)
     s32, t32, ev32 = run_timed(vb, p32, phi, args.warmup, args.steps)
     value = n ** 3 * ev32 / t32
@@ -399,6 +412,8 @@ def main():
     if not args.no_fp64:
         s64.close()
     if args.batch_steps > 0:
+        for size, _ in BATCH_GROUPS:
+            tune(size)
         bt = batched(vb, args.batch_steps)
         if "fp64" in out_extra:
             bt["ratio_vs_fp64_single_%d" % n] = bt["value"] / out_extra["fp64"]["value"]
@@ -417,6 +432,8 @@ def main():
                        "parallelism": "1 GPU"},
             "roofline": roof, "cpu_baseline": cpu}
     line.update(out_extra)
+    if tuned:
+        line["autotune_fp32"] = tuned
     print(json.dumps(line))
 
 

@@ -74,6 +74,14 @@ class Result(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MAX_SCHEDULES = 4
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n", C.c_int), ("precision", C.c_int), ("steps", C.c_int), ("candidates", C.c_int),
+                ("chosen", C.c_int), ("ms_per_step", C.c_double * MAX_SCHEDULES)]
+
+
 _LIB = None
 
 
@@ -158,6 +166,11 @@ def lib():
     L.vb_calc_etd42.argtypes = [P, P, P, P, D, D]
     L.vb_calc_etd43.argtypes = [P, P, P, P, D, D]
     L.vb_calc_etd44.argtypes = [P, P, P, P, P, P, D, D]
+    L.vb_schedule_count.argtypes = [I, I]
+    L.vb_schedule_get.argtypes = [I, I]
+    L.vb_schedule_set.argtypes = [I, I, I]
+    L.vb_schedule_describe.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    L.vb_autotune.argtypes = [I, I, I, I, C.POINTER(TuneReport)]
     _LIB = L
     return L
 
@@ -501,3 +514,34 @@ class Calc:
 
 def version():
     return lib().vb_version().decode()
+
+
+# ---- kernel schedules and the autotuner (vb_schedule_*, vb_autotune) --------
+def schedule_count(n, precision):
+    return int(lib().vb_schedule_count(int(n), int(precision)))
+
+
+def schedule_get(n, precision):
+    return int(lib().vb_schedule_get(int(n), int(precision)))
+
+
+def schedule_set(n, precision, sched):
+    _check(lib().vb_schedule_set(int(n), int(precision), int(sched)), "vb_schedule_set")
+
+
+def schedule_describe(n, precision, sched):
+    """{z_tile, x_passes, x_radix} of one compiled schedule."""
+    zt, xp, xr = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().vb_schedule_describe(int(n), int(precision), int(sched), C.byref(zt), C.byref(xp), C.byref(xr)),
+           "vb_schedule_describe")
+    return {"z_tile": zt.value, "x_passes": xp.value, "x_radix": xr.value}
+
+
+def autotune(n, precision, steps=4, device=0):
+    """Times every schedule of (n, precision) on the device, selects the fastest
+    and returns the report as a dict."""
+    r = TuneReport()
+    _check(lib().vb_autotune(int(n), int(precision), int(steps), int(device), C.byref(r)), "vb_autotune")
+    return {"n": r.n, "precision": r.precision, "steps": r.steps, "candidates": r.candidates,
+            "chosen": r.chosen, "ms_per_step": [r.ms_per_step[i] for i in range(r.candidates)],
+            "schedules": [schedule_describe(n, precision, i) for i in range(r.candidates)]}

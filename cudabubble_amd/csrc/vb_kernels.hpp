@@ -51,6 +51,9 @@ template <typename T, int N> constexpr int colw() {
 // their inputs are group-major, so only their P, Q outputs are written in
 // 64-byte pieces.
 template <typename T, int N> constexpr int tbw() { return (sizeof(T) == 4 && N == 256) ? 8 : colw<T, N>(); }
+// The other z tile a kernel schedule can select: the column width where tbw
+// is narrower, else half of it.
+template <typename T, int N> constexpr int tbw_alt() { return tbw<T, N>() < colw<T, N>() ? colw<T, N>() : tbw<T, N>() / 2; }
 
 // Z_p = A_ext(p) + i * B_ext(p): Hermitian extension along x of two half
 // spectra.  The imaginary parts at kx = 0 and kx = N/2 are dropped, which is
@@ -267,9 +270,12 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
 // transform instead of two; 204 registers, two waves per SIMD) ran 71 -> 65 us
 // against 8 x 8 x 4 (116 registers, four waves).
 template <typename T, int N> constexpr int xrplan() { return (sizeof(T) == 4 && N == 256) ? 1 : xplan<T, N>(); }
+// the other plan a kernel schedule can select (the same where no 16 x 16
+// plan exists)
+template <typename T, int N> constexpr int xrplan_alt() { return xrplan<T, N>() == 1 ? 0 : (N == 256 ? 1 : xrplan<T, N>()); }
 // threads of a k_xrow workgroup: as many rows as the other x kernels'
 // workgroups (two per row pair), so the partial sums line up
-template <typename T, int N> constexpr int xrthreads() { return 2 * xlanes<T, N>() * TFof<N, xrplan<T, N>()>(); }
+template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() { return 2 * xlanes<T, N>() * TFof<N, XP>(); }
 
 // The x kernel of an ETD stage, one row per lane: the x-inverse of P + i Q of
 // the row gives v and lap v; f, tmp1 and s follow pointwise; the x-forward of
@@ -286,11 +292,11 @@ template <typename T, int N> constexpr int xrthreads() { return 2 * xlanes<T, N>
 // where tmp1 is O(1), and the rounding of a shared transform scales with the
 // larger part: s goes in as eps^3 s, of tmp1's size, and its spectrum is
 // scaled back at the store.
-template <typename T, int N, bool TAB = false>
-__global__ void __launch_bounds__((xrthreads<T, N>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
+template <typename T, int N, bool TAB = false, int XP = xrplan<T, N>()>
+__global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
-  constexpr int XP = xrplan<T, N>(), E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(),
-                LL = row_lds_len<N>(), NT = xrthreads<T, N>();
+  constexpr int E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(), LL = row_lds_len<N>(),
+                NT = xrthreads<T, N, XP>();
   static_assert(NT % 64 == 0, "whole waves");
   __shared__ cpx<T> sm[RL * LL];
   __shared__ double red[3 * (NT / 64)];

@@ -54,6 +54,9 @@ template <typename T> struct ZsArgs;
 template <typename T>
 struct SizeOps {
   int N;
+  int sched;       // kernel schedule (z tile, x plan) this table launches
+  int xr_passes;   // passes of the ETD x kernel's plan
+  int xr_radix;    // radix of its first pass
   int xlanes;      // row pairs per x workgroup
   int ytile;       // columns per workgroup of the z stage kernels
   int ctile;       // columns per workgroup of the y passes and the z Laplacian
@@ -79,24 +82,51 @@ struct SizeOps {
   void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);
 };
 
-template <typename T, int N> const SizeOps<T>* ops_for();
+// launch table of kernel schedule sc of length N (nullptr if not compiled),
+// and the number of schedules compiled for it
+template <typename T, int N> const SizeOps<T>* ops_for(int sc);
+template <typename T, int N> int sched_count();
 
 #define VB_SIZE_LIST(X) X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 #define VB_DECL(n)                                              \
-  template <> const SizeOps<double>* ops_for<double, n>();      \
-  template <> const SizeOps<float>* ops_for<float, n>();
+  template <> const SizeOps<double>* ops_for<double, n>(int);   \
+  template <> const SizeOps<float>* ops_for<float, n>(int);     \
+  template <> int sched_count<double, n>();                     \
+  template <> int sched_count<float, n>();
 VB_SIZE_LIST(VB_DECL)
 #undef VB_DECL
 
 template <typename T>
-inline const SizeOps<T>* get_ops(int N) {
+inline int sched_count_of(int N) {
   switch (N) {
-#define VB_CASE(n) case n: return ops_for<T, n>();
+#define VB_CASE(n) case n: return sched_count<T, n>();
+    VB_SIZE_LIST(VB_CASE)
+#undef VB_CASE
+    default: return 0;
+  }
+}
+
+// The schedule in use for (precision, N): the autotuner's or the caller's
+// choice (vb_schedule_set), else 0; a thread's override (>= 0) wins while
+// set (the autotuner builds its candidate solvers under one).
+int sched_selected(bool fp32, int N);
+extern thread_local int g_sched_override;
+
+template <typename T>
+inline const SizeOps<T>* get_ops(int N, int sc) {
+  switch (N) {
+#define VB_CASE(n) case n: return ops_for<T, n>(sc);
     VB_SIZE_LIST(VB_CASE)
 #undef VB_CASE
     default: return nullptr;
   }
+}
+template <typename T>
+inline const SizeOps<T>* get_ops(int N) {
+  int sc = g_sched_override >= 0 ? g_sched_override : sched_selected(sizeof(T) == 4, N);
+  if (sc >= sched_count_of<T>(N)) sc = 0;
+  return get_ops<T>(N, sc);
 }
 
 inline bool size_supported(int N) {

@@ -8,11 +8,21 @@
 
 namespace vb {
 
+// Kernel schedules of one length: bit 0 selects the other z tile (tbw_alt),
+// bit 1 the other plan of the ETD x kernel (xrplan_alt).  Only FP32 has
+// alternatives; a schedule is compiled where its choice differs from the
+// default (see schedules()).
 template <typename T, int N>
+constexpr int schedules() {
+  return sizeof(T) == 8 ? 1 : (xrplan_alt<T, N>() != xrplan<T, N>() ? 4 : 2);
+}
+
+template <typename T, int N, int SC = 0>
 struct Launch {
   static constexpr int TF = TFof<N>();
   static constexpr int G = xlanes<T, N>();
-  static constexpr int TB = tbw<T, N>();
+  static constexpr int TB = (SC & 1) ? tbw_alt<T, N>() : tbw<T, N>();
+  static constexpr int XR = (SC & 2) ? xrplan_alt<T, N>() : xrplan<T, N>();
   static constexpr int YT = TB * TF;
   static constexpr int CV = colplan<T, N>();          // plan of the y passes and the z Laplacian
   static constexpr int CB = colw<T, N>();             // their columns per workgroup
@@ -41,7 +51,7 @@ struct Launch {
   }
   static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
     // k_xrow: one row per lane, over the same rows per workgroup
-    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? xrthreads<T, N>() : XWG);
+    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? xrthreads<T, N, XR>() : XWG);
     if (g_recorder) {
       const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
       return record_launch(&launch_tab, op, a, gr, bl);
@@ -49,7 +59,7 @@ struct Launch {
     if (mode == XA_WITHF)
       hipLaunchKernelGGL((k_xa<T, N, true>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_WITHS)
-      hipLaunchKernelGGL((k_xrow<T, N>), gr, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_xrow<T, N, false, XR>), gr, bl, 0, s, a, nullptr);
     else if (mode == XA_SUMS)
       hipLaunchKernelGGL((k_xsums<T, N>), gr, bl, 0, s, a, nullptr);
     else
@@ -82,9 +92,9 @@ struct Launch {
   static unsigned zgrid(const Geo& g) { return unsigned(zgroups(g, TB)); }
   static void zpass(cpx<T>* b, int sign, const Geo& g, const cpx<T>* tw, hipStream_t s) {
     if (sign < 0)
-      hipLaunchKernelGGL((k_zpass<T, N, -1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
+      hipLaunchKernelGGL((k_zpass<T, N, -1, TB>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
     else
-      hipLaunchKernelGGL((k_zpass<T, N, +1>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
+      hipLaunchKernelGGL((k_zpass<T, N, +1, TB>), dim3(zgrid(g)), dim3(YT), 0, s, b, g, tw);
   }
   static void zlap(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s) {
     const int nf = fin_groups(fin);
@@ -98,7 +108,7 @@ struct Launch {
     const dim3 gr(zgrid(g)), bl(YT);
     ZprArgs<T> a{vh, P, Q, part, dc, tw, g};
     if (g_recorder) return record_launch(&launch_tab, OP_ZPREP, a, gr, bl);
-    hipLaunchKernelGGL((k_zprep<T, N>), gr, bl, 0, s, a, nullptr);
+    hipLaunchKernelGGL((k_zprep<T, N, TB>), gr, bl, 0, s, a, nullptr);
   }
   template <int SCH, int STG, bool TAB>
   static void zs_launch(dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
@@ -134,7 +144,7 @@ struct Launch {
       case OP_XA: hipLaunchKernelGGL((k_xa<T, N, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_S:
-        hipLaunchKernelGGL((k_xrow<T, N, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        hipLaunchKernelGGL((k_xrow<T, N, true, XR>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
         break;
       case OP_XA_SUMS:
         hipLaunchKernelGGL((k_xsums<T, N, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
@@ -167,6 +177,9 @@ struct Launch {
   static SizeOps<T> table() {
     SizeOps<T> o;
     o.N = N;
+    o.sched = SC;
+    o.xr_passes = Plan<N, XR>::NP;
+    o.xr_radix = Plan<N, XR>::R[0];
     o.xlanes = G;
     o.ytile = TB;
     o.ctile = CB;
@@ -189,14 +202,22 @@ struct Launch {
 
 }  // namespace vb
 
-#define VB_DEFINE_SIZE(n)                                              \
-  namespace vb {                                                       \
-  template <> const SizeOps<double>* ops_for<double, n>() {            \
-    static const SizeOps<double> t = Launch<double, n>::table();       \
-    return &t;                                                         \
-  }                                                                    \
-  template <> const SizeOps<float>* ops_for<float, n>() {              \
-    static const SizeOps<float> t = Launch<float, n>::table();         \
-    return &t;                                                         \
-  }                                                                    \
+namespace vb {
+// the launch table of schedule sc of length N (nullptr: not compiled)
+template <typename T, int N>
+const SizeOps<T>* ops_table(int sc) {
+  if (sc < 0 || sc >= schedules<T, N>()) return nullptr;
+  static const SizeOps<T> t[4] = {
+      Launch<T, N, 0>::table(), Launch<T, N, (schedules<T, N>() > 1 ? 1 : 0)>::table(),
+      Launch<T, N, (schedules<T, N>() > 2 ? 2 : 0)>::table(), Launch<T, N, (schedules<T, N>() > 2 ? 3 : 0)>::table()};
+  return &t[sc];
+}
+}  // namespace vb
+
+#define VB_DEFINE_SIZE(n)                                                                     \
+  namespace vb {                                                                              \
+  template <> const SizeOps<double>* ops_for<double, n>(int sc) { return ops_table<double, n>(sc); } \
+  template <> const SizeOps<float>* ops_for<float, n>(int sc) { return ops_table<float, n>(sc); }    \
+  template <> int sched_count<double, n>() { return schedules<double, n>(); }                 \
+  template <> int sched_count<float, n>() { return schedules<float, n>(); }                   \
   }

@@ -110,9 +110,15 @@ def traffic(fetch, write, cal_fetch, cal_write, n, cmds, json_dir=None, json_pre
            "for a plain %d MiB device copy into the bytes it moves." % (ff, wf, COPY_BYTES >> 20), "",
            "Commands (repository root):", ""] + ["    " + c for c in cmds] + [
            "", "| kernel | precision | launches | read MB | written MB | total MB |", "|---|---|---:|---:|---:|---:|"]
+    # the JSON of a kernel is the row of its most-launched instantiation at
+    # length n (other lengths and kernel schedules appear in the table only)
+    main_row = {}
+    for r in rows:
+        if kernel_size(r[5]) == str(n) and ((r[0], r[1]) not in main_row or r[2] > main_row[(r[0], r[1])][2]):
+            main_row[(r[0], r[1])] = r
     for name, prec, cnt, rb, wb, k in sorted(rows, key=lambda r: -(r[3] + r[4])):
         out.append("| %s | %s | %d | %.1f | %.1f | %.1f |" % (name, prec, cnt, rb / 1e6, wb / 1e6, (rb + wb) / 1e6))
-        if json_dir and (json_prec is None or prec == json_prec):
+        if json_dir and (json_prec is None or prec == json_prec) and main_row.get((name, prec), (0,) * 6)[5] == k:
             with open(os.path.join(json_dir, "traffic_%s_%d_%s.json" % (name, n, prec)), "w") as fh:
                 json.dump({"kernel": name, "n": n, "precision": prec, "launches": cnt,
                            "read_bytes_per_launch": rb, "write_bytes_per_launch": wb,

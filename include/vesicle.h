@@ -247,6 +247,35 @@ double vb_server_clock(vb_server* s);
 /* finishes every queued and running request, then frees everything */
 void vb_server_destroy(vb_server* s);
 
+/* ---- kernel schedules and the autotuner -------------------------------------
+ * No reference counterpart: cuFFT picks its own plans inside the reference's
+ * Calc (Calc.cu:58-64).  Each grid length is compiled with a few kernel
+ * schedules: the column tile of the z kernels and the radix plan of the ETD
+ * x kernel (FP32-mixed only; FP64 has one).  A process-wide selection per
+ * (length, precision), schedule 0 until set, applies to every solver, batch
+ * and Calc created afterwards; solves created under different selections
+ * differ in rounding only.  The schedule functions need no GPU. */
+#define VB_MAX_SCHEDULES 4
+typedef struct vb_tune_report {
+  int n, precision, steps;
+  int candidates;                          /* schedules compiled for (n, precision) */
+  int chosen;                              /* the fastest, now selected             */
+  double ms_per_step[VB_MAX_SCHEDULES];    /* best of 3 timings of `steps` driver steps */
+} vb_tune_report;
+/* schedules compiled for grid length n (0: length not supported) */
+int vb_schedule_count(int n, int precision);
+/* the selected schedule (-1: length not supported) */
+int vb_schedule_get(int n, int precision);
+int vb_schedule_set(int n, int precision, int sched);
+/* z_tile: columns per workgroup of the z kernels; x_passes, x_radix: passes
+ * of the ETD x kernel's FFT plan and the radix of its first pass */
+int vb_schedule_describe(int n, int precision, int sched, int* z_tile, int* x_passes, int* x_radix);
+/* Times every schedule of (n, precision) on the benchmark workload (n^3
+ * torus, eps = 2h, fixed-dt ETDRK4 with penalty drift, dt = 5e-7) on the
+ * device and selects the fastest.  Makes the device current on the calling
+ * thread; do not call it while other solves of that length are created. */
+int vb_autotune(int n, int precision, int steps, int device, vb_tune_report* rep);
+
 /* The ETD coefficient table the solver uses for step h (host only, no GPU):
  * out[m][8] for m = kx^2 + ky'^2 + kz'^2 in [0, mmax] with folded indices.
  * ETDRK4 records {e1, (e1-1)/H, e2, A s, 2 B s, C s, 0, H} (Calc.cu:495-567,

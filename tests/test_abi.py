@@ -184,3 +184,25 @@ def test_create_without_gpu_reports_no_device(vb):
     with pytest.raises(vb.VbError) as e:
         vb.Solver(p, np.zeros((32, 32, 32)))
     assert e.value.status == vb.VB_ERR_NO_DEVICE
+
+
+def test_schedules_without_gpu(vb):
+    # kernel schedules per grid length (vb_schedule_*): host-side tables only
+    assert vb.schedule_count(256, vb.FP32_MIXED) == 4
+    assert vb.schedule_count(128, vb.FP32_MIXED) == 2
+    assert vb.schedule_count(256, vb.FP64) == 1
+    assert vb.schedule_count(100, vb.FP32_MIXED) == 0 and vb.schedule_get(100, vb.FP32_MIXED) == -1
+    d = [vb.schedule_describe(256, vb.FP32_MIXED, s) for s in range(4)]
+    # the default: 8-column z tiles and the 16 x 16 plan of the ETD x kernel
+    assert d[0] == {"z_tile": 8, "x_passes": 2, "x_radix": 16}
+    assert len({tuple(x.values()) for x in d}) == 4
+    saved = vb.schedule_get(256, vb.FP32_MIXED)
+    try:
+        vb.schedule_set(256, vb.FP32_MIXED, 3)
+        assert vb.schedule_get(256, vb.FP32_MIXED) == 3
+        assert vb.schedule_get(256, vb.FP64) == 0
+        for bad in ((256, vb.FP64, 1), (256, vb.FP32_MIXED, 4), (100, vb.FP32_MIXED, 0), (256, 7, 0)):
+            with pytest.raises(vb.VbError):
+                vb.schedule_set(*bad)
+    finally:
+        vb.schedule_set(256, vb.FP32_MIXED, saved)
