@@ -249,7 +249,7 @@ void vb_server_destroy(vb_server* s);
 
 /* ---- kernel schedules and the autotuner -------------------------------------
  * No reference counterpart: cuFFT picks its own plans inside the reference's
- * Calc (Calc.cu:58-64).  Each grid length is compiled with a few kernel
+ * Calc (Calc.cu:56-58).  Each grid length is compiled with a few kernel
  * schedules: the column tile of the z kernels and the radix plan of the ETD
  * x kernel (FP32-mixed only; FP64 has one).  A process-wide selection per
  * (length, precision), schedule 0 until set, applies to every solver, batch
