@@ -109,7 +109,8 @@ def traffic(fetch, write, cal_fetch, cal_write, n, cmds, json_dir=None, json_pre
            "FETCH_SIZE scaled by %.3f and WRITE_SIZE by %.3f: the factors that turn each counter's reading "
            "for a plain %d MiB device copy into the bytes it moves." % (ff, wf, COPY_BYTES >> 20), "",
            "Commands (repository root):", ""] + ["    " + c for c in cmds] + [
-           "", "| kernel | precision | launches | read MB | written MB | total MB |", "|---|---|---:|---:|---:|---:|"]
+           "", "| kernel | precision | launches | read MB | written MB | total MB | template arguments |",
+           "|---|---|---:|---:|---:|---:|---|"]
     # the JSON of a kernel is the row of its most-launched instantiation at
     # length n (other lengths and kernel schedules appear in the table only)
     main_row = {}
@@ -117,7 +118,9 @@ def traffic(fetch, write, cal_fetch, cal_write, n, cmds, json_dir=None, json_pre
         if kernel_size(r[5]) == str(n) and ((r[0], r[1]) not in main_row or r[2] > main_row[(r[0], r[1])][2]):
             main_row[(r[0], r[1])] = r
     for name, prec, cnt, rb, wb, k in sorted(rows, key=lambda r: -(r[3] + r[4])):
-        out.append("| %s | %s | %d | %.1f | %.1f | %.1f |" % (name, prec, cnt, rb / 1e6, wb / 1e6, (rb + wb) / 1e6))
+        targs = re.search(r"vb::k_\w+(<[^>]*>)", k)
+        out.append("| %s | %s | %d | %.1f | %.1f | %.1f | `%s` |" % (name, prec, cnt, rb / 1e6, wb / 1e6, (rb + wb) / 1e6,
+                                                                  targs.group(1) if targs else "-"))
         if json_dir and (json_prec is None or prec == json_prec) and main_row.get((name, prec), (0,) * 6)[5] == k:
             with open(os.path.join(json_dir, "traffic_%s_%d_%s.json" % (name, n, prec)), "w") as fh:
                 json.dump({"kernel": name, "n": n, "precision": prec, "launches": cnt,
