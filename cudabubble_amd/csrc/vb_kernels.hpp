@@ -194,9 +194,9 @@ struct XaArgs {
 // Every pipeline kernel takes its argument struct by value, or (TAB, a
 // batched launch over the solves of a size group) from a per-solve table in
 // device memory indexed by blockIdx.z.
-// k_xa: the x kernel of the Euler paths and of the reference-dataflow
-// evaluation (tmp1 out, and with WITHF v, f and the x-forward of f), two rows
-// per lane.
+// k_xa: the x kernel of the Euler paths (WITHF: v, f and the x-forward of f
+// as well) and of the all-FP64 integrals of mixed mode; partial sums and the
+// x-forward of tmp1, two rows per lane.
 template <typename T, int N, bool WITHF, bool TAB = false>
 __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
