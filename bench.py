@@ -50,6 +50,8 @@ def alg_bytes(name, n, prec_bytes):
         # (state or Vh), stage/state spectra; next-stage field (Vh), P, Q out
         "zstage1": modes * (6 * cs + 16), "zstage2": modes * (7 * cs + 16),
         "zstage3": modes * 8 * cs, "zstage4": modes * (7 * cs + 32),
+        # stage 4 of a drift step's first evaluation: no state out, P alone
+        "zstage4p": modes * (6 * cs + 16),
     }
     return table.get(name)
 

@@ -705,18 +705,23 @@ __device__ __forceinline__ void next_field_elem(double re, double im, int kz, co
   yo = mk<T>(T(re * m), T(im * m));
 }
 
-template <typename T, int N, int TB, int V, class Lds>
+// PONLY: the z-inverse of the field alone into P (Q, the Laplacian, is not
+// needed); the Parseval partials are stored either way.
+template <typename T, int N, int TB, int V, bool PONLY = false, class Lds>
 __device__ __forceinline__ void next_field_tail(cpx<T>* x, cpx<T>* y, double acc, int j, const ZIdx& id,
                                                 const Geo& g, Lds& lds, Lds& lds2, const cpx<T>* __restrict__ tw,
                                                 cpx<T>* P, cpx<T>* Q, double* part, double* red) {
   constexpr int E = Plan<N, V>::E, TF = N / E;
   const long long plane = g.pp;
-  fft2<T, N, +1, V>(x, y, j, tw, lds, lds2);
+  if constexpr (PONLY)
+    fft<T, N, +1, V>(x, j, tw, lds);
+  else
+    fft2<T, N, +1, V>(x, y, j, tw, lds, lds2);
   if (id.valid) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       P[id.q + (long long)(j + TF * e) * plane] = x[e];
-      Q[id.q + (long long)(j + TF * e) * plane] = y[e];
+      if constexpr (!PONLY) Q[id.q + (long long)(j + TF * e) * plane] = y[e];
     }
   }
   double s[1] = {acc};
@@ -794,11 +799,17 @@ struct ZsArgs {
 // no minimum occupancy: a cap of three or four waves per SIMD spills heavily
 // and runs slower.  Every stream but P, Q (read by the next kernel) and the
 // coefficient tables is loaded and stored nontemporally (see ld_nt).
+// PROBE (ETDRK4 stage 4 only): the result is needed only for its V and A,
+// the first evaluation of a penalty-drift step (belastic.cpp:677-683): the
+// new state is not stored, and only P (its z-inverse) and the Parseval and
+// DC terms are.
 // Plan variant of the stage combines (see Plan).
 template <typename T, int N> constexpr int zsplan() { return 0; }
 
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>()>
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
+          bool PROBE = false>
 __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
+  static_assert(!PROBE || (SCH == 4 && STG == 4), "the probe form is the last ETDRK4 stage");
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, V>::E, TF = N / E;
   __shared__ cpx<T> sm[2 * N * TB];
@@ -900,7 +911,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
       // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
       fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
       fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
-      if (id.valid) st_nt(a.uout + ib, mk<double>(fr, fi));
+      if (id.valid && !PROBE) st_nt(a.uout + ib, mk<double>(fr, fi));
     } else if constexpr (SCH == 2 && STG == 1) {
       const double2 c = a.c1[m];                      // e^z, q1
       const cpx<double> u = ur[e];
@@ -925,7 +936,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
     next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
-  next_field_tail<T, N, TB, V>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB, V, PROBE>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
 // Copy of a half spectrum between layouts, widened to FP64: tb_in / tb_out =

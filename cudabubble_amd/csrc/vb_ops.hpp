@@ -18,7 +18,7 @@ namespace vb {
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
   OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_Y_FWD_GM, OP_ZLAP, OP_ZPREP,
-  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET, OP_ZS44P
 };
 // x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
 // the partial sums of v and (v^2-1)^2 only (from P alone: k_xsums)
@@ -79,6 +79,8 @@ struct SizeOps {
   void (*zlap)(cpx<T>* b, const Geo& g, const cpx<T>* tw, const FinArgs* fin, hipStream_t s);
   void (*zprep)(const cpx<double>* vh, cpx<T>* P, cpx<T>* Q, double* part, double* dc, const Geo& g, const cpx<T>* tw,
                 hipStream_t s);
+  // stg 1..sch; 5: the probe form of ETDRK4 stage 4 (V and A of the result
+  // only: no state store, P alone)
   void (*zstage)(const ZsArgs<T>& a, int sch, int stg, hipStream_t s);
 };
 

@@ -110,9 +110,10 @@ struct Launch {
     if (g_recorder) return record_launch(&launch_tab, OP_ZPREP, a, gr, bl);
     hipLaunchKernelGGL((k_zprep<T, N, TB>), gr, bl, 0, s, a, nullptr);
   }
-  template <int SCH, int STG, bool TAB>
+  template <int SCH, int STG, bool TAB, bool PROBE = false>
   static void zs_launch(dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
-    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB>), gr, dim3(TB * TFof<N, zsplan<T, N>()>()), 0, s, a, tab);
+    hipLaunchKernelGGL((k_zstage<T, N, SCH, STG, TB, TAB, zsplan<T, N>(), PROBE>), gr,
+                       dim3(TB * TFof<N, zsplan<T, N>()>()), 0, s, a, tab);
   }
   template <bool TAB>
   static void zs_dispatch(int sch, int stg, dim3 gr, const ZsArgs<T>& a, const ZsArgs<T>* tab, hipStream_t s) {
@@ -121,7 +122,8 @@ struct Launch {
         case 1: zs_launch<4, 1, TAB>(gr, a, tab, s); break;
         case 2: zs_launch<4, 2, TAB>(gr, a, tab, s); break;
         case 3: zs_launch<4, 3, TAB>(gr, a, tab, s); break;
-        default: zs_launch<4, 4, TAB>(gr, a, tab, s); break;
+        case 4: zs_launch<4, 4, TAB>(gr, a, tab, s); break;
+        default: zs_launch<4, 4, TAB, true>(gr, a, tab, s); break;   // 5: the probe form of stage 4
       }
     } else if (stg == 1) {
       zs_launch<2, 1, TAB>(gr, a, tab, s);
@@ -132,7 +134,7 @@ struct Launch {
   static void zstage(const ZsArgs<T>& a, int sch, int stg, hipStream_t s) {
     const dim3 gr(zgrid(a.g));
     if (g_recorder)
-      return record_launch(&launch_tab, (sch == 4 ? OP_ZS41 : OP_ZS21) + (stg - 1), a, gr,
+      return record_launch(&launch_tab, stg == 5 ? OP_ZS44P : (sch == 4 ? OP_ZS41 : OP_ZS21) + (stg - 1), a, gr,
                            dim3(TB * TFof<N, zsplan<T, N>()>()));
     zs_dispatch<false>(sch, stg, gr, a, nullptr, s);
   }
@@ -167,6 +169,7 @@ struct Launch {
       case OP_ZS41: case OP_ZS42: case OP_ZS43: case OP_ZS44:
         zs_dispatch<true>(4, op - OP_ZS41 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;
+      case OP_ZS44P: zs_dispatch<true>(4, 5, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s); break;
       case OP_ZS21: case OP_ZS22:
         zs_dispatch<true>(2, op - OP_ZS21 + 1, gr, ZsArgs<T>{}, (const ZsArgs<T>*)tab, s);
         break;

@@ -526,8 +526,10 @@ struct Solver final : SolverBase {
   }
 
   // The combine of stage stg: z-forward of tmp1 and s, N^, the ETD combine,
-  // and the z-inverse of the next stage field into P, Q.
-  void suffix(int sch, int stg, const Lut* lt, int out) {
+  // and the z-inverse of the next stage field into P, Q.  probe (last ETDRK4
+  // stage): the new state is only needed for its V and A, so neither it nor Q
+  // is stored.
+  void suffix(int sch, int stg, const Lut* lt, int out, bool probe = false) {
     ZsArgs<T> z;
     z.U = stg == 1 ? Tt.p : Tt1.p;
     z.U2 = stg == 1 ? Ts.p : Ts1.p;
@@ -550,11 +552,14 @@ struct Solver final : SolverBase {
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
-    L(names[stg], [&] { oz->zstage(z, sch, stg, st); });
+    if (probe && sch == 4 && stg == 4)
+      L("zstage4p", [&] { oz->zstage(z, sch, 5, st); });
+    else
+      L(names[stg], [&] { oz->zstage(z, sch, stg, st); });
     check_launch();
     prefixed = false;
     prefixed_f = false;
-    prepped = true;   // P, Q hold the next stage field (see below)
+    prepped = !(probe && sch == 4 && stg == 4);   // P, Q hold the next stage field (the probe: P alone)
   }
 
   void ensure_prefixed(bool withf = false) {
@@ -569,15 +574,16 @@ struct Solver final : SolverBase {
   }
 
   // one ETDRK4 / ETDRK2 evaluation from uh[cur] into uh[1-cur]; on return P, Q
-  // hold the prepared new state.
-  int eval(double h) {
+  // hold the prepared new state.  probe: the new state is needed only for V
+  // and A (prefix_integrals); for ETDRK4 uh[1-cur] and Q are then not written.
+  int eval(double h, bool probe = false) {
     const int sch = p.scheme == VB_ETDRK2 ? 2 : 4;
     const Lut* lt = lut(sch, h);
     const int out = 1 - cur;
     ensure_prefixed();
     for (int stg = 1; stg <= sch; ++stg) {
       if (stg > 1) prefix(FIN_PLAIN, false, 1);
-      suffix(sch, stg, lt, out);
+      suffix(sch, stg, lt, out, probe);
     }
     // after the last stage P, Q belong to uh[out]
     return out;
@@ -622,7 +628,7 @@ struct Solver final : SolverBase {
     if (drift_mode()) {
       ensure_prefixed();
       launch_keep(OP_KEEP, scal.p, scal_keep.p, ph, st);
-      int out = eval(time_step);
+      int out = eval(time_step, true);
       const int keep = cur;
       cur = out;
       prefix_integrals(FIN_DRIFT);   // only the drift of the targets is used

@@ -52,7 +52,9 @@ def solver_name(mangled):
         return ("y_fwd" if a[1] == "-1" else "y_inv"), prec
     if kind == "zpass":
         return ("z_fwd" if a[1] == "-1" else "z_inv"), prec
-    if kind == "zstage":
+    if kind == "zstage":   # <T, N, SCH, STG, TB, TAB, V, PROBE>
+        if a[6] == "true":
+            return "zstage4p", prec
         return "zstage%s_etdrk%s" % (a[2], a[1]) if a[1] != "4" else "zstage%s" % a[2], prec
     if kind == "zlap":
         return "z_lap", prec
