@@ -315,13 +315,16 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const double v = double(x[e].x), lv = double(x[e].y);
-    const double w = v * v - 1.0;
-    const double f = -lv * eps + w * v * ieps;                          // f_kernel, Calc.cu:160
-    const double t1 = v * (v * v - 1.0 - kappa);                        // nld_kernel1, Calc.cu:270
-    const double sv = D * t1 + (3 * v * v - D - 1) * eps * f;        // eps^3 s, Calc.cu:282
-    if (valid) { acc[0] += v; acc[1] += w * w; acc[2] += f * f; }
-    x[e] = mk<T>(T(t1), T(sv));
+    // in the work precision: v and lap v come out of a T transform, so an
+    // FP64 evaluation would add nothing to their rounding (measured: the same
+    // errors against the FP64 run, 48 fewer registers); the sums are FP64
+    const T v = x[e].x, lv = x[e].y;
+    const T w = v * v - T(1);
+    const T f = -lv * T(eps) + w * v * T(ieps);                         // f_kernel, Calc.cu:160
+    const T t1 = v * (w - T(kappa));                                    // nld_kernel1, Calc.cu:270
+    const T sv = T(D) * t1 + (T(3) * v * v - T(D + 1)) * (T(eps) * f);  // eps^3 s, Calc.cu:282
+    if (valid) { acc[0] += double(v); acc[1] += double(w) * double(w); acc[2] += double(f) * double(f); }
+    x[e] = mk<T>(t1, sv);
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
   unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
