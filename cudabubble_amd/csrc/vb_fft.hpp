@@ -63,9 +63,6 @@ template <> struct Plan<384, 0> { static constexpr int E = 12, NP = 5; static co
 template <> struct Plan<512, 0> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 template <int N> struct Plan<N, 1> : Plan<N, 0> {};
 template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 16}; };
-// V = 2 trades passes for registers: 256 = 4 * 4 * 4 * 4 over 64 threads
-template <int N> struct Plan<N, 2> : Plan<N, 0> {};
-template <> struct Plan<256, 2> { static constexpr int E = 4, NP = 4; static constexpr int R[4] = {4, 4, 4, 4}; };
 
 template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 

@@ -806,7 +806,9 @@ struct ZsArgs {
 // the first evaluation of a penalty-drift step (belastic.cpp:677-683): the
 // new state is not stored, and only P (its z-inverse) and the Parseval and
 // DC terms are.
-// Plan variant of the stage combines (see Plan).
+// Plan variant of the stage combines (see Plan): the default plan.  The
+// 16 x 16 plan needs 248-256 registers there and ran slower (stage 4 158 ->
+// 182 us), and so did 4 x 4 x 4 x 4 over 64 threads (100-128 registers).
 template <typename T, int N> constexpr int zsplan() { return 0; }
 
 template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
