@@ -15,7 +15,6 @@
 
 #include <chrono>
 #include <cmath>
-#include <cstdio>
 #include <mutex>
 #include <vector>
 
