@@ -799,9 +799,10 @@ struct ZsArgs {
 // spectrum in Vh.  The N^ of a stage is stored in precision T for a later
 // stage; the state and every combine that produces it are FP64.
 // All stages load their operands after the forward transforms, and ask for
-// no minimum occupancy: a cap of three or four waves per SIMD spills heavily
-// and runs slower.  Every stream but P, Q (read by the next kernel) and the
-// coefficient tables is loaded and stored nontemporally (see ld_nt).
+// no minimum occupancy but the probe's (zswaves): a cap of three or four
+// waves per SIMD spills heavily and runs slower.  Every stream but P, Q (read
+// by the next kernel) and the coefficient tables is loaded and stored
+// nontemporally (see ld_nt).
 // PROBE (ETDRK4 stage 4 only): the result is needed only for its V and A,
 // the first evaluation of a penalty-drift step (belastic.cpp:677-683): the
 // new state is not stored, and only P (its z-inverse) and the Parseval and
@@ -811,9 +812,15 @@ struct ZsArgs {
 // 182 us), and so did 4 x 4 x 4 x 4 over 64 threads (100-128 registers).
 template <typename T, int N> constexpr int zsplan() { return 0; }
 
+// Minimum waves per SIMD of a stage combine: three for the probe form in FP32
+// at 256 (4 spilled registers, 131 -> 125 us); the full stage 4 under the
+// same cap spills 18 and ran 155 -> 197 us.
+template <typename T, int N, bool PROBE> constexpr int zswaves() { return (sizeof(T) == 4 && N == 256 && PROBE) ? 3 : 1; }
+
 template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
           bool PROBE = false>
-__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
+__global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>())) k_zstage(ZsArgs<T> a0,
+                                                                                          const ZsArgs<T>* tab) {
   static_assert(!PROBE || (SCH == 4 && STG == 4), "the probe form is the last ETDRK4 stage");
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, V>::E, TF = N / E;
