@@ -334,6 +334,10 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
 // The partial sums of v and (v^2-1)^2 from P alone, for the targets' drift in
 // the first half of the penalty-drift double step (it needs V and A only, not
 // E_b; the sum of f^2 is left zero).  Two rows of P travel as one transform.
+// A workgroup covers kXsLoop row groups of the other x kernels in turn (group
+// blockIdx.x + k gridDim.x), so the finalize reads kXsLoop times fewer
+// partials; v^2 - 1 is formed in the work precision, like v, the sums in FP64.
+constexpr int kXsLoop = 4;   // 1, 2, 4, 8: sums + finalize 37, 34, 32, 37 us at 256^3
 template <typename T, int N, bool TAB = false>
 __global__ void __launch_bounds__(XWG) k_xsums(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
@@ -342,22 +346,27 @@ __global__ void __launch_bounds__(XWG) k_xsums(XaArgs<T> a0, const XaArgs<T>* ta
   __shared__ double red[3 * (XWG / 64)];
   const int lane = threadIdx.x / TF, j = threadIdx.x % TF;
   RowLds<T> lds{sm + lane * LL};
-  long long pair = (long long)blockIdx.x * G + lane;
-  const bool valid = pair < a.npairs;
-  if (!valid) pair = a.npairs - 1;
-  const long long r0 = 2 * pair;
-  cpx<T> x[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(r0), a.P + a.rl.off(r0 + 1), j + TF * e);
-  fft<T, N, +1, XP>(x, j, a.tw, lds);
+  const long long ngroups = (a.npairs + G - 1) / G;
   double acc[3] = {0.0, 0.0, 0.0};
+  // the bound is the same for every thread of the workgroup (the transform
+  // has barriers)
+  for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    long long pair = grp * G + lane;
+    const bool valid = pair < a.npairs;
+    if (!valid) pair = a.npairs - 1;
+    const long long r0 = 2 * pair;
+    cpx<T> x[E];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+    for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(r0), a.P + a.rl.off(r0 + 1), j + TF * e);
+    fft<T, N, +1, XP>(x, j, a.tw, lds);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const double v = double(h ? x[e].y : x[e].x);
-      const double w = v * v - 1.0;
-      if (valid) { acc[0] += v; acc[1] += w * w; }
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const T v = h ? x[e].y : x[e].x;
+        const double w = double(v * v - T(1));
+        if (valid) { acc[0] += double(v); acc[1] += w * w; }
+      }
     }
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);

@@ -50,8 +50,10 @@ struct Launch {
     hipLaunchKernelGGL((k_xc2r<T, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, rows(g));
   }
   static void xa(const XaArgs<T>& a, int mode, hipStream_t s) {
-    // k_xrow: one row per lane, over the same rows per workgroup
-    const dim3 gr(xgrid(a.npairs)), bl(mode == XA_WITHS ? xrthreads<T, N, XR>() : XWG);
+    // k_xrow: one row per lane, over the same rows per workgroup; k_xsums:
+    // kXsLoop row groups per workgroup
+    const dim3 gr(mode == XA_SUMS ? (xgrid(a.npairs) + kXsLoop - 1) / kXsLoop : xgrid(a.npairs)),
+        bl(mode == XA_WITHS ? xrthreads<T, N, XR>() : XWG);
     if (g_recorder) {
       const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
       return record_launch(&launch_tab, op, a, gr, bl);

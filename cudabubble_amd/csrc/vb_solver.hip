@@ -514,14 +514,16 @@ struct Solver final : SolverBase {
     a.inv_n = 1.0 / double(g.nr);
     a.rl = Rows{g.ny, g.pitch, g.pp};
     L("xa_sums", [&] { ox->xa(a, XA_SUMS, st); });
-    refinalize(mode);
+    refinalize(mode, (nxb + kXsLoop - 1) / kXsLoop);
     prepped = false;
     prefixed = false;
     prefixed_f = false;
   }
 
-  void refinalize(int mode) {
-    L("finalize", [&] { launch_finalize(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
+  // nx_parts: the x partials written by the last x kernel (k_xsums writes fewer)
+  void refinalize(int mode, int nx_parts = -1) {
+    const int nxp = nx_parts < 0 ? nxb : nx_parts;
+    L("finalize", [&] { launch_finalize(xpart.p, nxp, zpart.p, dc.p, nzb, scal.p, g, ph, mode, st); });
     check_launch();
   }
 
