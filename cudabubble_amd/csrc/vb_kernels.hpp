@@ -875,86 +875,94 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>()))
     np[e] = mk<double>(pr, pi);
   }
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
-  cpx<double> ur[USES_U ? E : 1];
-  cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? E : 1];
-  cpx<T> pb[(SCH == 4 && STG == 4) ? E : 1];
-  cpx<T> pv[STG > 1 ? E : 1];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const long long ib = ib0 + (long long)(j + TF * e) * TB;
-    if constexpr (USES_U) ur[e] = id.valid ? ld_nt(a.uh + ib) : mk<double>(0.0, 0.0);
-    if constexpr (SCH == 4 && STG == 3) pa[e] = ld_nt(a.pre + ib);
-    if constexpr (SCH == 4 && STG == 4) { pa[e] = ld_nt(a.Na + ib); pb[e] = ld_nt(a.Nb + ib); }
-    if constexpr (SCH == 2 && STG == 2) pa[e] = ld_nt(a.Na + ib);
-    if constexpr (STG > 1) pv[e] = ld_nt(a.Vh + ib);
-  }
+  // FP64 stage 4 loads and combines its operands in two halves: all at once
+  // they need 256 registers and 38 more in AGPRs, one wave per SIMD
+  constexpr int NH = (sizeof(T) == 8 && SCH == 4 && STG == 4) ? 2 : 1, EH = E / NH;
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int kz = j + TF * e;
-    const long long ib = ib0 + (long long)kz * TB;
-    const int m = mindex(g, id.kx, id.ky, kz);
-    const double av = -l2c * (a.eps * double(m) * g.kfac2 + a.kappa * ie);
-    double vr, vi;   // the stage field's spectrum
-    if constexpr (STG == 1) {
-      vr = ur[e].x;
-      vi = ur[e].y;
-    } else {
-      vr = double(pv[e].x);
-      vi = double(pv[e].y);
+  for (int hh = 0; hh < NH; ++hh) {
+    cpx<double> ur[USES_U ? EH : 1];
+    cpx<T> pa[(SCH == 4 && STG >= 3) || (SCH == 2 && STG == 2) ? EH : 1];
+    cpx<T> pb[(SCH == 4 && STG == 4) ? EH : 1];
+    cpx<T> pv[STG > 1 ? EH : 1];
+#pragma unroll
+    for (int iq = 0; iq < EH; ++iq) {
+      const int e = hh * EH + iq;
+      const long long ib = ib0 + (long long)(j + TF * e) * TB;
+      if constexpr (USES_U) ur[iq] = id.valid ? ld_nt(a.uh + ib) : mk<double>(0.0, 0.0);
+      if constexpr (SCH == 4 && STG == 3) pa[iq] = ld_nt(a.pre + ib);
+      if constexpr (SCH == 4 && STG == 4) { pa[iq] = ld_nt(a.Na + ib); pb[iq] = ld_nt(a.Nb + ib); }
+      if constexpr (SCH == 2 && STG == 2) pa[iq] = ld_nt(a.Na + ib);
+      if constexpr (STG > 1) pv[iq] = ld_nt(a.Vh + ib);
     }
-    const double nr = np[e].x + av * vr, ni = np[e].y + av * vi;   // N^ of this stage
-    x[e] = mk<T>(T(nr), T(ni));
-    double fr, fi;   // next field
-    if constexpr (SCH == 4 && STG == 1) {
-      const double2 c = a.c1[m];                      // e1, q
-      const cpx<double> u = ur[e];
-      fr = c.x * u.x + c.y * nr;                      // etd41: a^
-      fi = c.x * u.y + c.y * ni;
-      if (id.valid) st_nt(a.pre + ib, mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni)));
-    } else if constexpr (SCH == 4 && STG == 2) {
-      const double2 c = a.c1[m];
-      const cpx<double> u = ur[e];
-      fr = c.x * u.x + c.y * nr;                      // etd42: b^
-      fi = c.x * u.y + c.y * ni;
-      if (id.valid) st_nt(a.Na + ib, x[e]);
-    } else if constexpr (SCH == 4 && STG == 3) {
-      const double q = a.c1[m].y;
-      const cpx<T> p = pa[e];
-      fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
-      fi = double(p.y) + 2.0 * q * ni;
-      if (id.valid) st_nt(a.Nb + ib, x[e]);
-    } else if constexpr (SCH == 4 && STG == 4) {
-      const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
-      const cpx<double> u = ur[e];
-      const cpx<T> na = pa[e], nb = pb[e];
-      // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
-      fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
-      fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
-      if (id.valid && !PROBE) st_nt(a.uout + ib, mk<double>(fr, fi));
-    } else if constexpr (SCH == 2 && STG == 1) {
-      const double2 c = a.c1[m];                      // e^z, q1
-      const cpx<double> u = ur[e];
-      fr = c.x * u.x + c.y * nr;                      // etd1: a^
-      fi = c.x * u.y + c.y * ni;
-      if (id.valid) st_nt(a.Na + ib, x[e]);
-    } else {                                          // SCH == 2 && STG == 2
-      const double2 c = a.c1[m];
-      const double c2 = a.c2[m].x;
-      const cpx<double> u = ur[e];
-      const cpx<T> nu = pa[e];
-      // etd2: u^+ = F(a) + c2 (N^a - N^u), a^ = e^z u^ + q1 N^u
-      const double ar = c.x * u.x + c.y * double(nu.x), ai = c.x * u.y + c.y * double(nu.y);
-      fr = ar + c2 * (nr - double(nu.x));
-      fi = ai + c2 * (ni - double(nu.y));
-      if (id.valid) st_nt(a.uout + ib, mk<double>(fr, fi));
+#pragma unroll
+    for (int iq = 0; iq < EH; ++iq) {
+      const int e = hh * EH + iq;
+      const int kz = j + TF * e;
+      const long long ib = ib0 + (long long)kz * TB;
+      const int m = mindex(g, id.kx, id.ky, kz);
+      const double av = -l2c * (a.eps * double(m) * g.kfac2 + a.kappa * ie);
+      double vr, vi;   // the stage field's spectrum
+      if constexpr (STG == 1) {
+        vr = ur[iq].x;
+        vi = ur[iq].y;
+      } else {
+        vr = double(pv[iq].x);
+        vi = double(pv[iq].y);
+      }
+      const double nr = np[e].x + av * vr, ni = np[e].y + av * vi;   // N^ of this stage
+      x[e] = mk<T>(T(nr), T(ni));
+      double fr, fi;   // next field
+      if constexpr (SCH == 4 && STG == 1) {
+        const double2 c = a.c1[m];                      // e1, q
+        const cpx<double> u = ur[iq];
+        fr = c.x * u.x + c.y * nr;                      // etd41: a^
+        fi = c.x * u.y + c.y * ni;
+        if (id.valid) st_nt(a.pre + ib, mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni)));
+      } else if constexpr (SCH == 4 && STG == 2) {
+        const double2 c = a.c1[m];
+        const cpx<double> u = ur[iq];
+        fr = c.x * u.x + c.y * nr;                      // etd42: b^
+        fi = c.x * u.y + c.y * ni;
+        if (id.valid) st_nt(a.Na + ib, x[e]);
+      } else if constexpr (SCH == 4 && STG == 3) {
+        const double q = a.c1[m].y;
+        const cpx<T> p = pa[iq];
+        fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
+        fi = double(p.y) + 2.0 * q * ni;
+        if (id.valid) st_nt(a.Nb + ib, x[e]);
+      } else if constexpr (SCH == 4 && STG == 4) {
+        const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
+        const cpx<double> u = ur[iq];
+        const cpx<T> na = pa[iq], nb = pb[iq];
+        // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
+        fr = c.x * u.x + (double(na.x) * c.y + (double(na.x) + double(nb.x)) * c.z + nr * c.w);
+        fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
+        if (id.valid && !PROBE) st_nt(a.uout + ib, mk<double>(fr, fi));
+      } else if constexpr (SCH == 2 && STG == 1) {
+        const double2 c = a.c1[m];                      // e^z, q1
+        const cpx<double> u = ur[iq];
+        fr = c.x * u.x + c.y * nr;                      // etd1: a^
+        fi = c.x * u.y + c.y * ni;
+        if (id.valid) st_nt(a.Na + ib, x[e]);
+      } else {                                          // SCH == 2 && STG == 2
+        const double2 c = a.c1[m];
+        const double c2 = a.c2[m].x;
+        const cpx<double> u = ur[iq];
+        const cpx<T> nu = pa[iq];
+        // etd2: u^+ = F(a) + c2 (N^a - N^u), a^ = e^z u^ + q1 N^u
+        const double ar = c.x * u.x + c.y * double(nu.x), ai = c.x * u.y + c.y * double(nu.y);
+        fr = ar + c2 * (nr - double(nu.x));
+        fi = ai + c2 * (ni - double(nu.y));
+        if (id.valid) st_nt(a.uout + ib, mk<double>(fr, fi));
+      }
+      if constexpr (STG < SCH) {
+        if (id.valid) st_nt(a.Vh + ib, mk<T>(T(fr), T(fi)));
+      }
+      if (e == 0) dc0 = fr;
+      next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
     }
-    if constexpr (STG < SCH) {
-      if (id.valid) st_nt(a.Vh + ib, mk<T>(T(fr), T(fi)));
-    }
-    if (e == 0) dc0 = fr;
-    next_field_elem<T>(fr, fi, kz, id, g, acc, x[e], y[e]);
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
   next_field_tail<T, N, TB, V, PROBE>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
