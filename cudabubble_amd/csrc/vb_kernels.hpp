@@ -266,10 +266,11 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
-// Plan of k_xrow: in FP32 at 256 the 16 x 16 plan (one LDS exchange per
-// transform instead of two; 204 registers, two waves per SIMD) ran 71 -> 65 us
-// against 8 x 8 x 4 (116 registers, four waves).
-template <typename T, int N> constexpr int xrplan() { return (sizeof(T) == 4 && N == 256) ? 1 : xplan<T, N>(); }
+// Plan of k_xrow: at 256 the 16 x 16 plan (one LDS exchange per transform
+// instead of two) ran 71 -> 65 us in FP32 (then 204 registers, two waves per
+// SIMD, against 116 and four for 8 x 8 x 4) and raised the FP64 rate 3 % (250
+// registers, two waves).
+template <typename T, int N> constexpr int xrplan() { return N == 256 ? 1 : xplan<T, N>(); }
 // the other plan a kernel schedule can select (the same where no 16 x 16
 // plan exists)
 template <typename T, int N> constexpr int xrplan_alt() { return xrplan<T, N>() == 1 ? 0 : (N == 256 ? 1 : xrplan<T, N>()); }
@@ -875,9 +876,9 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>()))
     np[e] = mk<double>(pr, pi);
   }
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
-  // FP64 stage 4 loads and combines its operands in two halves: all at once
-  // they need 256 registers and 38 more in AGPRs, one wave per SIMD
-  constexpr int NH = (sizeof(T) == 8 && SCH == 4 && STG == 4) ? 2 : 1, EH = E / NH;
+  // The last FP64 stage loads and combines its operands in two halves: all at
+  // once they need 256 registers and up to 38 more in AGPRs, one wave per SIMD
+  constexpr int NH = (sizeof(T) == 8 && STG == SCH) ? 2 : 1, EH = E / NH;
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll
