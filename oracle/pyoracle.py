@@ -61,6 +61,7 @@ def lib():
         L.vbo_set_targets.argtypes = [P, D, D]
         L.vbo_get_params.argtypes = [P, C.POINTER(Params)]
         L.vbo_threads.restype = I
+        L.vbo_set_threads.argtypes = [I]
         for name, n in [("vbo_r2c", 2), ("vbo_c2r", 2), ("vbo_lap_spec", 2), ("vbo_lap_real", 2),
                         ("vbo_fpf", 2), ("vbo_g0", 2), ("vbo_sum_fgq", 2)]:
             getattr(L, name).argtypes = [P] + [P] * n

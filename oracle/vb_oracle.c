@@ -171,6 +171,16 @@ int vbo_threads(void) {
 #endif
 }
 
+/* threads of the following parallel regions (n >= 1); the results do not
+ * depend on it (fixed reduction splits) */
+void vbo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n >= 1) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* 3-D transforms.  Forward: D2Z then scale by 1/(nx ny nz) (Calc.cu:88-96). */
 /* Backward: Z2D, unnormalised (Calc.cu:99-104).                            */

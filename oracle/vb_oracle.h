@@ -46,6 +46,7 @@ void vbo_free(vbo_ctx* c);
 void vbo_set_targets(vbo_ctx* c, double alpha, double beta);
 void vbo_get_params(const vbo_ctx* c, vbo_params* out);
 int vbo_threads(void);
+void vbo_set_threads(int n);
 
 /* ---- transforms (Calc.cu:77-104) ------------------------------------- */
 void vbo_r2c(vbo_ctx* c, const double* u, double* s);   /* normalised   */
