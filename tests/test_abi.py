@@ -206,3 +206,22 @@ def test_schedules_without_gpu(vb):
                 vb.schedule_set(*bad)
     finally:
         vb.schedule_set(256, vb.FP32_MIXED, saved)
+
+
+def test_autotune_arguments_and_no_device(vb):
+    # invalid lengths, precisions and step counts are rejected before any device work
+    for args in ((100, vb.FP32_MIXED), (64, 7)):
+        with pytest.raises(vb.VbError) as e:
+            vb.autotune(*args)
+        assert e.value.status == vb.VB_ERR_PARAM
+    with pytest.raises(vb.VbError) as e:
+        vb.autotune(64, vb.FP32_MIXED, steps=0)
+    assert e.value.status == vb.VB_ERR_PARAM
+    import torch
+    if torch.cuda.is_available():
+        return
+    saved = vb.schedule_get(32, vb.FP32_MIXED)
+    with pytest.raises(vb.VbError) as e:
+        vb.autotune(32, vb.FP32_MIXED, steps=1)
+    assert e.value.status == vb.VB_ERR_NO_DEVICE
+    assert vb.schedule_get(32, vb.FP32_MIXED) == saved   # a failed tuning selects nothing
