@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def restore(vb):
-    saved = {(n, pr): vb.schedule_get(n, pr) for n in (32, 64, 256) for pr in (0, 1)}
+    saved = {(n, pr): vb.schedule_get(n, pr) for n in (32, 64, 128, 192, 256) for pr in (0, 1)}
     yield
     for (n, pr), sc in saved.items():
         vb.schedule_set(n, pr, sc)
@@ -40,7 +40,7 @@ def solve(vb, p, phi, steps, graphs=1):
     return out
 
 
-@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("n", [32, 64, 128, 192, 256])
 def test_every_schedule_computes_the_same_solve(vb, gpu, restore, n):
     steps = 5
     p, phi = params(vb, n, vb.FP64)
