@@ -101,3 +101,24 @@ def test_precision_case_256(vb, gpu):
     # difference ranges from 5e-8 to 2e-5 while phi stays within 3e-7; E also
     # carries the drifted targets of each run
     assert rel(q32[3], q64[3]) < 3e-5 and rel(q32[0], q64[0]) < 1e-3
+
+
+@pytest.mark.parametrize("n", [384, 512])
+def test_fp32_mixed_large(vb, gpu, n):
+    # the FP32-mixed pipeline at the large sizes against the FP64 one on the same
+    # field: three penalty-drift steps (six evaluations, the probe stage included)
+    eps = 2.0 * 2 * np.pi / n
+    phi, _ = vb.shape(vb.SHAPE_TORUS, 20260003, n, n, 1.0, eps, noise=1e-3)
+    out = {}
+    for prec in (vb.FP64, vb.FP32_MIXED):
+        flush_print("fp32-mixed large", n, prec)
+        p = vb.params_default(nx=n, nz=n, eps=eps, M1=1e4, M2=1e4, alpha=0.0, beta=0.0, precision=prec,
+                              phi_lut_mode=vb.LUT_ACCURATE, time_step=5e-7, fixed_time_step=5e-7)
+        s = vb.Solver(p, phi)
+        r = s.step(3)
+        assert r["status"] == 0 and r["evaluations"] == 6
+        out[prec] = (s.get_phi(), s.query())
+        s.close()
+    (p64, q64), (p32, q32) = out[vb.FP64], out[vb.FP32_MIXED]
+    assert rel_l2(p32, p64) < 1e-5                                  # the FP32-mixed bound on phi
+    assert rel(q32[1], q64[1]) < 1e-6 and rel(q32[2], q64[2]) < 1e-6   # V, A
