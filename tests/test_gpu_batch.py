@@ -129,3 +129,27 @@ def test_size_group_launches_match_solo(vb, gpu):
         assert got[i][1] == ref_q, i
         assert got[i][2]["time_current"] == ref_state["time_current"], i
         assert reps[i]["evaluations"] == ref_reps[0]["evaluations"], i
+
+
+def test_batch_large_sizes_match_solo(vb, gpu):
+    # the 128^3 and 192^3 size groups of the batched configuration: each solve of
+    # a shared launch computes exactly what it computes alone
+    specs = [dict(nx=128, kind=1, seed=90, M1=1e4, M2=3e4), dict(nx=128, kind=3, seed=91, M1=3e3, M2=1e4),
+             dict(nx=192, kind=4, seed=92, M1=1e4, M2=1e4)]
+    ps, phis = [], []
+    for sp in specs:
+        sp = dict(sp)
+        p, phi = case(vb, sp.pop("nx"), None, sp.pop("kind"), sp.pop("seed"), precision=1, phi_lut_mode=1,
+                      time_step=1e-7, fixed_time_step=1e-7, **sp)
+        ps.append(p)
+        phis.append(phi)
+    flush_print("large batch of", len(ps))
+    b = vb.Batch(ps, phis)
+    reps = b.step(4)
+    got = [(b.get_phi(i), b.query(i)) for i in range(len(ps))]
+    b.close()
+    for i, (p, phi) in enumerate(zip(ps, phis)):
+        assert reps[i]["status"] == 0 and reps[i]["evaluations"] == 8, i
+        ref_phi, _, ref_q, _ = solo(vb, p, phi, [4], graphs=0)
+        assert np.array_equal(got[i][0], ref_phi), i
+        assert got[i][1] == ref_q, i
