@@ -17,6 +17,8 @@
 // The Euler paths keep a real-space right-hand side: their x kernel (k_xa)
 // stores v and f and transforms f, and y, z(lap), y kernels take lap f back to x-spectra.
 #pragma once
+#include <type_traits>
+
 #include "vb_device.hpp"
 
 namespace vb {
@@ -572,8 +574,7 @@ struct ZIdx {
 // t: the workgroup's column group, blockIdx.x unless the launch has extra
 // workgroups in front
 template <int TB>
-__device__ __forceinline__ ZIdx zidx(const Geo& g, long long t) {
-  const int c = threadIdx.x % TB;
+__device__ __forceinline__ ZIdx zidx(const Geo& g, long long t, int c) {
   ZIdx r;
   if (g.aligned) {
     const int tpr = (g.nx / 2) / TB;
@@ -599,6 +600,10 @@ __device__ __forceinline__ ZIdx zidx(const Geo& g, long long t) {
   }
   r.q = (long long)r.ky * g.pitch + r.kx;
   return r;
+}
+template <int TB>
+__device__ __forceinline__ ZIdx zidx(const Geo& g, long long t) {
+  return zidx<TB>(g, t, int(threadIdx.x % TB));
 }
 template <int TB>
 __device__ __forceinline__ ZIdx zidx(const Geo& g) {
@@ -809,8 +814,9 @@ struct ZsArgs {
 // spectrum in Vh.  The N^ of a stage is stored in precision T for a later
 // stage; the state and every combine that produces it are FP64.
 // All stages load their operands after the forward transforms, and ask for
-// no minimum occupancy but the probe's (zswaves): a cap of three or four
-// waves per SIMD spills heavily and runs slower.  Every stream but P, Q (read
+// no minimum occupancy: a cap of three or four waves per SIMD spills heavily
+// and runs slower (the probe form under a three-wave cap ran faster with
+// per-element gathers, slower with the staged table below).  Every stream but P, Q (read
 // by the next kernel) and the coefficient tables is loaded and stored
 // nontemporally (see ld_nt).
 // PROBE (ETDRK4 stage 4 only): the result is needed only for its V and A,
@@ -822,19 +828,25 @@ struct ZsArgs {
 // 182 us), and so did 4 x 4 x 4 x 4 over 64 threads (100-128 registers).
 template <typename T, int N> constexpr int zsplan() { return 0; }
 
-// Minimum waves per SIMD of a stage combine: three for the probe form in FP32
-// at 256 (4 spilled registers, 131 -> 125 us); the full stage 4 under the
-// same cap spills 18 and ran 155 -> 197 us.
-template <typename T, int N, bool PROBE> constexpr int zswaves() { return (sizeof(T) == 4 && N == 256 && PROBE) ? 3 : 1; }
-
 template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
           bool PROBE = false>
-__global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>())) k_zstage(ZsArgs<T> a0,
-                                                                                          const ZsArgs<T>* tab) {
+__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
   static_assert(!PROBE || (SCH == 4 && STG == 4), "the probe form is the last ETDRK4 stage");
   const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, V>::E, TF = N / E;
-  __shared__ cpx<T> sm[2 * N * TB];
+  // ETDRK4 stages 1, 3 and 4 read their coefficient records from a
+  // per-workgroup table in LDS: TB columns x (N/2 + 1) folded kz, gathered once
+  // after the forward transforms.  Gathered per element, the 64 lanes of a load
+  // touched up to 64 cache lines and the combines issued 4x the L1-to-L2
+  // requests of their streams; staged, stages 1, 3, 4 and the probe ran 120,
+  // 123, 161, 126 -> 112, 107, 148, 122 us at 256^3 FP32.  Stage 2 keeps the
+  // gathers: the table cost it a wave per SIMD and time.
+  constexpr bool STAGED = SCH == 4 && STG != 2;
+  using Rec = typename std::conditional<STG == 4, double4, double2>::type;
+  constexpr int NQ = N / 2 + 1;
+  constexpr int FFTB = int(2 * N * TB * sizeof(cpx<T>)), TBLB = STAGED ? int(TB * NQ * sizeof(Rec)) : 0;
+  __shared__ __align__(16) unsigned char smraw[FFTB > TBLB ? FFTB : TBLB];
+  cpx<T>* sm = reinterpret_cast<cpx<T>*>(smraw);
   __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
@@ -875,6 +887,19 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>()))
     if (id.q == 0 && kz == 0) pr -= l1h;
     np[e] = mk<double>(pr, pi);
   }
+  const Rec* tbl = reinterpret_cast<const Rec*>(smraw);
+  if constexpr (STAGED) {
+    // the transforms' last LDS reads are behind their last barrier
+    Rec* w = reinterpret_cast<Rec*>(smraw);
+    for (int i = threadIdx.x; i < TB * NQ; i += TB * TF) {
+      const int c = i / NQ, q = i % NQ;
+      const ZIdx ic = zidx<TB>(g, blockIdx.x, c);
+      const int m = mindex(g, ic.kx, ic.ky, q);
+      if constexpr (STG == 4) w[i] = a.c2[m]; else w[i] = a.c1[m];
+    }
+    __syncthreads();
+  }
+  const int tcol = int(threadIdx.x % TB);
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
   // The last FP64 stage loads and combines its operands in two halves: all at
   // once they need 256 registers and up to 38 more in AGPRs, one wave per SIMD
@@ -916,25 +941,25 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>()))
       x[e] = mk<T>(T(nr), T(ni));
       double fr, fi;   // next field
       if constexpr (SCH == 4 && STG == 1) {
-        const double2 c = a.c1[m];                      // e1, q
+        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c1[m];   // e1, q
         const cpx<double> u = ur[iq];
         fr = c.x * u.x + c.y * nr;                      // etd41: a^
         fi = c.x * u.y + c.y * ni;
         if (id.valid) st_nt(a.pre + ib, mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni)));
       } else if constexpr (SCH == 4 && STG == 2) {
-        const double2 c = a.c1[m];
+        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c1[m];
         const cpx<double> u = ur[iq];
         fr = c.x * u.x + c.y * nr;                      // etd42: b^
         fi = c.x * u.y + c.y * ni;
         if (id.valid) st_nt(a.Na + ib, x[e]);
       } else if constexpr (SCH == 4 && STG == 3) {
-        const double q = a.c1[m].y;
+        const double q = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)].y : a.c1[m].y;
         const cpx<T> p = pa[iq];
         fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
         fi = double(p.y) + 2.0 * q * ni;
         if (id.valid) st_nt(a.Nb + ib, x[e]);
       } else if constexpr (SCH == 4 && STG == 4) {
-        const double4 c = a.c2[m];                      // e2, A s, 2 B s, C s
+        const double4 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c2[m];   // e2, A s, 2 B s, C s
         const cpx<double> u = ur[iq];
         const cpx<T> na = pa[iq], nb = pb[iq];
         // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
@@ -966,6 +991,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), (zswaves<T, N, PROBE>()))
     }
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
+  if constexpr (STAGED) __syncthreads();   // the table's last reads before the transforms reuse LDS
   next_field_tail<T, N, TB, V, PROBE>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
 }
 
