@@ -57,12 +57,8 @@ def alg_bytes(name, n, prec_bytes):
 
 
 def config_params(vb, n, precision, lut_mode):
-    eps = 2.0 * (2 * np.pi) / n
-    phi, sp = vb.shape(vb.SHAPE_TORUS, 20260003, n, n, 1.0, eps, noise=1e-3)
-    p = vb.params_default(nx=n, nz=n, eps=eps, M1=1e4, M2=1e4, alpha=0.0, beta=0.0,
-                          time_step=5e-7, fixed_time_step=5e-7, precision=precision,
-                          phi_lut_mode=lut_mode, scheme=vb.ETDRK4)
-    return p, phi, sp
+    from cudabubble_amd.workloads import precision_case
+    return precision_case(vb, n, precision, lut_mode)
 
 
 def sync():
@@ -138,49 +134,13 @@ def cpu_baseline(n, seconds=10.0):
                             "sample": "1 ETDRK4 evaluation of the same field on one thread, %.1f s" % t1}}
 
 
-BATCH_GROUPS = ((32, 16), (64, 16), (128, 12), (192, 4))    # (n, solves)
-
-
-def bending_energy(phi, eps, L):
-    """E_b = (1/2eps) int f^2, f = -eps lap phi + (phi^2-1) phi / eps, by numpy FFT
-    (host-side set-up of the batched form only)."""
-    n = phi.shape[-1]
-    k = np.fft.fftfreq(n, 1.0 / n) * (2 * np.pi / L)
-    kz = np.fft.fftfreq(phi.shape[0], 1.0 / phi.shape[0]) * (2 * np.pi / L)
-    k2 = kz[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2
-    lap = np.real(np.fft.ifftn(-k2 * np.fft.fftn(phi)))
-    f = -eps * lap + (phi * phi - 1) * phi / eps
-    return float(np.mean(f * f)) * L ** 3 / (2 * eps)
-
-
 def batched(vb, steps, seed=20260004):
     """The batched form: 48 FP32-mixed ETDRK4 solves of uneven size resident
-    together on the GPU (16 x 32^3, 16 x 64^3, 12 x 128^3, 4 x 192^3).  Solve j
-    is an ellipsoid, torus or biconcave disk (in turn) drawn by vb_shape from
-    seed + j; eps in {1.5h, 2h, 3h} and M1, M2 log-uniform in [1e3, 1e5] come
-    from numpy's PCG64(seed); fixed dt with the penalty drift (two evaluations
-    per step).  The penalties enter N explicitly, so their stiffness bounds dt:
-    the volume mode relaxes at rate M1 L^3 / 4 and the area mode at about
-    M2 (9/8) 2 eps E_b, and the reference scheme diverges once dt times their
-    sum exceeds about 1 (torus and disk shapes at M2 ~ 1e5 have E_b ~ 1e2-1e3).
-    Each solve takes dt = min(1e-7, 0.5 / that sum) from its initial field."""
-    rng = np.random.default_rng(seed)
-    ps, phis = [], []
-    j = 0
-    for n, count in BATCH_GROUPS:
-        for _ in range(count):
-            kind = (vb.SHAPE_ELLIPSOID, vb.SHAPE_TORUS, vb.SHAPE_DISK)[j % 3]
-            eps = (1.5, 2.0, 3.0)[int(rng.integers(3))] * (2 * np.pi) / n
-            M1, M2 = 10.0 ** rng.uniform(3, 5, size=2)
-            phi, _ = vb.shape(kind, seed + j, n, n, 1.0, eps)
-            L = 2 * np.pi
-            stiff = M1 * L ** 3 / 4 + M2 * 9.0 / 8.0 * 2 * eps * bending_energy(phi, eps, L)
-            dt = min(1e-7, 0.5 / stiff)
-            ps.append(vb.params_default(nx=n, nz=n, eps=eps, M1=M1, M2=M2, alpha=0.0, beta=0.0,
-                                        time_step=dt, fixed_time_step=dt, precision=vb.FP32_MIXED,
-                                        phi_lut_mode=vb.LUT_ACCURATE, scheme=vb.ETDRK4))
-            phis.append(phi)
-            j += 1
+    together on the GPU (16 x 32^3, 16 x 64^3, 12 x 128^3, 4 x 192^3), fixed dt
+    with the penalty drift; fields, parameters and per-solve dt as in
+    cudabubble_amd.workloads.batch_mix."""
+    from cudabubble_amd.workloads import BATCH_GROUPS, batch_mix
+    ps, phis = batch_mix(vb, seed)
     b = vb.Batch(ps, phis)
     b.step(2)     # start-up: alpha/beta, tables, graph capture
     sync()
@@ -197,42 +157,44 @@ def batched(vb, steps, seed=20260004):
             "groups": ["%d x %d^3" % (c, n) for n, c in BATCH_GROUPS]}
 
 
-SERVE_SIZES = ((64, 0.50), (128, 0.35), (256, 0.14), (512, 0.01))
+def converge_run(vb, p, phi, window=50, tol=1e-6):
+    """One request alone through vb_step, with the server's stopping rule:
+    relative energy drop over `window` accepted steps below tol, t >= tmax or a
+    driver exit.  Returns the solver and the accepted dt of every step."""
+    s = vb.Solver(p, phi)
+    hist, dts = [], []
+    while True:
+        r = s.step(1)
+        if r["status"] != 0:
+            break
+        dts.append(r["time_step"])
+        hist.append(r["energy"])
+        if r["time_current"] >= p.tmax:
+            break
+        if len(hist) > window and abs(hist[-window - 1] - hist[-1]) <= tol * abs(hist[-window - 1]):
+            break
+    return s, dts
 
 
-def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05):
+def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05, sample_cap=None):
     """The serving form: Poisson arrivals at `rate` requests/s for `seconds`,
     grid sizes drawn from {64^3: 50%, 128^3: 35%, 256^3: 14%, 512^3: 1%},
     shapes and parameters drawn as in the batched form, FP32-mixed adaptive
     ETDRK4 from dt = 1e-7 until the relative energy drop over 50 accepted steps
-    is below 1e-6 or t >= tmax = 1e-3.  Latency is submit -> result.  A 5%
-    sample is re-run in FP64 (same request, one at a time) for the error."""
-    rng = np.random.default_rng(seed)
-    t_arr, reqs = 0.0, []
-    while True:
-        t_arr += rng.exponential(1.0 / rate)
-        if t_arr > seconds:
-            break
-        n = int(rng.choice([s for s, _ in SERVE_SIZES], p=[q for _, q in SERVE_SIZES]))
-        j = len(reqs)
-        kind = (vb.SHAPE_ELLIPSOID, vb.SHAPE_TORUS, vb.SHAPE_DISK)[j % 3]
-        eps = (1.5, 2.0, 3.0)[int(rng.integers(3))] * (2 * np.pi) / n
-        M1, M2 = 10.0 ** rng.uniform(3, 5, size=2)
-        reqs.append((t_arr, n, kind, eps, M1, M2))
+    is below 1e-6 or t >= tmax = 1e-3.  Latency is submit -> result.
 
-    def make(j, precision):
-        _, n, kind, eps, M1, M2 = reqs[j]
-        phi, _ = vb.shape(kind, seed + j, n, n, 1.0, eps)
-        p = vb.params_default(nx=n, nz=n, eps=eps, M1=M1, M2=M2, alpha=0.0, beta=0.0, time_step=1e-7,
-                              fixed_time_step=0.0, precision=precision, phi_lut_mode=vb.LUT_ACCURATE,
-                              scheme=vb.ETDRK4, tmax=1e-3)
-        return p, phi
+    A 5% sample is re-run in FP64, each request alone, two ways: on the FP32-mixed
+    run's own accepted dt sequence (vb_evaluate at each accepted dt from the
+    same start: the two differ by rounding only), and with its own adaptive dt
+    (the FP64 path can accept and reject differently near an equilibrium)."""
+    from cudabubble_amd.workloads import serve_request, serve_stream
+    reqs, rng = serve_stream(vb, seconds, rate, seed)
 
     srv = vb.Server(max_active=16, streams=4, converge_window=50, converge_tol=1e-6)
     results, t0 = {}, time.perf_counter()
-    for j, (ta, *_rest) in enumerate(reqs):
-        p, phi = make(j, vb.FP32_MIXED)      # prepared before its arrival time
-        while time.perf_counter() - t0 < ta:
+    for j, req in enumerate(reqs):
+        p, phi = serve_request(vb, req, j, vb.FP32_MIXED, seed)     # prepared before its arrival time
+        while time.perf_counter() - t0 < req[0]:
             for r in srv.poll():
                 results[r["id"]] = r
             time.sleep(0.0005)
@@ -245,43 +207,49 @@ def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05):
     wait = np.array([r["t_admit"] - r["t_submit"] for r in results.values()])
     gp_evals = sum(reqs[i][1] ** 3 * r["evaluations"] for i, r in results.items())
     span = max(r["t_done"] for r in results.values()) - min(r["t_submit"] for r in results.values())
-    # FP64 re-runs of a sample, each alone
     sample = sorted(rng.choice(len(reqs), size=max(1, int(round(fp64_sample * len(reqs)))), replace=False))
+    if sample_cap:
+        sample = sample[:sample_cap]
     errs = []
     for j in sample:
-        p, phi = make(int(j), vb.FP64)
-        s = vb.Solver(p, phi)
-        steps = 0
-        hist = []
-        while True:
-            r = s.step(1)
-            steps += 1
-            hist.append(r["energy"])
-            if r["status"] != 0 or r["time_current"] >= p.tmax:
-                break
-            if len(hist) > 50 and abs(hist[-51] - hist[-1]) <= 1e-6 * abs(hist[-51]):
-                break
-        E, V, A, Eb = s.query()
-        s.close()
-        q = results[int(j)]
-        errs.append({"id": int(j), "n": reqs[int(j)][1], "E_rel": abs(q["E"] - E) / abs(E),
-                     "V_rel": abs(q["V"] - V) / abs(V), "A_rel": abs(q["A"] - A) / abs(A),
-                     "steps_fp32": q["steps"], "steps_fp64": steps})
+        j = int(j)
+        q = results[j]
+        p32, phi = serve_request(vb, reqs[j], j, vb.FP32_MIXED, seed)
+        s32, dts = converge_run(vb, p32, phi)
+        alone = s32.query()
+        s32.close()
+        p64, _ = serve_request(vb, reqs[j], j, vb.FP64, seed)
+        m = vb.Solver(p64, phi)
+        m.step(0)                          # the driver start: alpha/beta from the field
+        for h in dts:
+            m.evaluate(h)
+        Em, Vm, Am, Ebm = m.query()
+        m.close()
+        s64, dts64 = converge_run(vb, p64, phi)
+        E, V, A, Eb = s64.query()
+        s64.close()
+        errs.append({"id": j, "n": reqs[j][1], "steps_fp32": q["steps"], "steps_fp64": len(dts64),
+                     "served_equals_alone": [q["E"], q["V"], q["A"], q["Eb"]] == list(alone),
+                     "matched_path": {"E_rel": abs(q["E"] - Em) / abs(Em), "V_rel": abs(q["V"] - Vm) / abs(Vm),
+                                      "A_rel": abs(q["A"] - Am) / abs(Am), "Eb_rel": abs(q["Eb"] - Ebm) / abs(Ebm)},
+                     "own_path": {"E_rel": abs(q["E"] - E) / abs(E), "V_rel": abs(q["V"] - V) / abs(V),
+                                  "A_rel": abs(q["A"] - A) / abs(A)}})
     sizes = {}
     for i in results:
         sizes[reqs[i][1]] = sizes.get(reqs[i][1], 0) + 1
     status = {}
     for r in results.values():
         status[str(r["status"])] = status.get(str(r["status"]), 0) + 1
+    worst = {k: max(e["matched_path"][k] for e in errs) for k in ("E_rel", "V_rel", "A_rel", "Eb_rel")} if errs else {}
     return {"requests": len(reqs), "completed": sum(1 for r in results.values() if r["status"] == 0),
-            "status_counts": status,
-            "arrival_seconds": seconds, "rate_per_s": rate, "sizes": {str(k): v for k, v in sorted(sizes.items())},
+            "status_counts": status, "arrival_seconds": seconds, "rate_per_s": rate,
+            "sizes": {str(k): v for k, v in sorted(sizes.items())},
             "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "max": float(lat.max())},
             "queue_wait_s": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
             "aggregate_gp_steps_per_s": gp_evals / span, "wall_s": wall,
             "mean_steps": float(np.mean([r["steps"] for r in results.values()])),
-            "fp64_sample": errs}
+            "fp64_sample_worst_matched_path": worst, "fp64_sample": errs}
 
 
 def copy_calibration():
@@ -332,7 +300,12 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=300,
                     help="steps of the 48-solve batched form (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
-                    help="arrival window of the serving form at 20 requests/s (0: skip it)")
+                    help="arrival window of the serving form at 20 requests/s (0: skip it); the "
+                         "configuration states 120 s, the default is a reduced window")
+    ap.add_argument("--serve-sample-cap", type=int, default=0,
+                    help="at most this many requests of the 5%% FP64 sample (0: all of them)")
+    ap.add_argument("--precision-steps", type=int, default=500,
+                    help="driver steps after which the FP32-mixed state is compared with the FP64 one")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep schedule 0 instead of autotuning the kernel schedules of each grid size")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -372,10 +345,13 @@ def main():
         p64, _, _ = config_params(vb, n, vb.FP64, vb.LUT_ACCURATE)
         s64, t64, ev64 = run_timed(vb, p64, phi, args.warmup, args.steps)
         v64 = n ** 3 * ev64 / t64
-        # both solvers have now taken warmup + steps + latency + profile steps? align them
+        # both solvers continue (untimed) to the precision configuration's length
         done32 = args.warmup + args.steps + args.latency_steps + 2
         done64 = args.warmup + args.steps
-        s64.step(done32 - done64)
+        done = max(done32, args.precision_steps)
+        s32.step(done - done32)
+        s64.step(done - done64)
+        done32 = done
         phi32, phi64 = s32.get_phi(), s64.get_phi()
         E32, V32, A32, Eb32 = s32.query()
         E64, V64, A64, Eb64 = s64.query()
@@ -400,8 +376,13 @@ def main():
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None,
             "traffic": None, "avg_launch_us": avg * 1e6, "alg_bytes_per_launch": b}
-    copy_bw = copy_bandwidth()
+    # the streaming ceiling: the library's own 16-byte copy kernels (best variant)
+    # and torch's copy, each over 1 GiB
+    copy_bw, copy_each = vb.copy_bandwidth(1 << 30, repeats=20)
+    copy_bw *= 1e9
     roof["copy_GBps"] = copy_bw / 1e9
+    roof["copy_variants_GBps"] = copy_each
+    roof["copy_GBps_torch"] = copy_bandwidth() / 1e9
     roof["frac_of_copy"] = achieved / copy_bw if achieved else None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%d_fp32.json" % (name, n))
     if os.path.exists(tfile):
@@ -412,8 +393,26 @@ def main():
     R4 = n ** 3 * 4
     B_alg = 35 * C4 + 44 * R4
     t_eval = t32 / ev32
-    out_extra["step_roofline"] = {"B_alg_per_evaluation": B_alg, "achieved_GBps": B_alg / t_eval / 1e9,
-                                  "frac": B_alg / t_eval / HBM_PEAK, "frac_of_copy": B_alg / t_eval / copy_bw}
+    # A penalty-drift step is two evaluations, and `value` counts both; the bytes
+    # it needs are fewer: the second evaluation reuses the first stage's prefix
+    # of the state (one N-evaluation, 5 C + 11 R) and the first evaluation's
+    # result is used only for V and A (no state write, C)
+    B_step = 2 * B_alg - (5 * C4 + 11 * R4) - C4
+    t_step = t32 / args.steps
+    out_extra["step_roofline"] = {
+        "B_alg_per_evaluation": B_alg, "achieved_GBps": B_alg / t_eval / 1e9,
+        "frac": B_alg / t_eval / HBM_PEAK, "frac_of_copy": B_alg / t_eval / copy_bw,
+        "drift_step": {"B_needed": B_step, "achieved_GBps": B_step / t_step / 1e9, "frac": B_step / t_step / HBM_PEAK,
+                       "note": "2 evaluations per drift step less the reused stage-1 prefix and the probe's "
+                               "unwritten state"}}
+    # the plain fixed-dt ETDRK4 step (targets fixed, no drift): one evaluation per step
+    pp, _, _ = config_params(vb, n, vb.FP32_MIXED, vb.LUT_ACCURATE)
+    pp.penalty_drift = 0
+    sp_, tp, evp = run_timed(vb, pp, phi, args.warmup, args.steps)
+    sp_.close()
+    out_extra["plain_etdrk4_fp32"] = {"value": n ** 3 * evp / tp, "ms_per_step": tp * 1e3 / args.steps,
+                                      "evaluations_per_step": evp / args.steps,
+                                      "frac": B_alg / (tp / evp) / HBM_PEAK}
     out_extra["latency_ms_per_step"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                                         "n": len(lat)}
     out_extra["kernels_fp32"] = [{"name": nm, "launches": c, "ms": round(m, 3),
@@ -424,6 +423,7 @@ def main():
     if not args.no_fp64:
         s64.close()
     if args.batch_steps > 0:
+        from cudabubble_amd.workloads import BATCH_GROUPS
         for size, _ in BATCH_GROUPS:
             tune(size)
         bt = batched(vb, args.batch_steps)
@@ -431,14 +431,18 @@ def main():
             bt["ratio_vs_fp64_single_%d" % n] = bt["value"] / out_extra["fp64"]["value"]
         out_extra["batched"] = bt
     if args.serve_seconds > 0:
-        out_extra["serving"] = serving(vb, args.serve_seconds)
+        sv = serving(vb, args.serve_seconds, sample_cap=args.serve_sample_cap or None)
+        if args.serve_seconds < 120:
+            sv["reduced"] = "%g s of arrivals instead of the configuration's 120 s" % args.serve_seconds
+        out_extra["serving"] = sv
     cpu = None if args.no_cpu else cpu_baseline(n)
 
     line = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "%d^3 torus (seed 20260003, noise 1e-3), eps=2h, ETDRK4 fixed dt=5e-7, "
-                                   "M1=M2=1e4 penalty drift (2 evaluations per step), FP32-mixed" % n,
+                                   "M1=M2=1e4 penalty drift (2 evaluations per step, both counted in value: "
+                                   "the first only feeds the drift of the targets), FP32-mixed" % n,
                        "grid": n, "evaluations_per_step": ev32 / args.steps, "shape": [round(x, 6) for x in sp[:2]],
                        "lut": "accurate", "precision": "fp32-mixed (f64 tables, reductions, penalty/dt logic)",
                        "parallelism": "1 GPU"},

@@ -171,6 +171,7 @@ def lib():
     L.vb_schedule_set.argtypes = [I, I, I]
     L.vb_schedule_describe.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     L.vb_autotune.argtypes = [I, I, I, I, C.POINTER(TuneReport)]
+    L.vb_copy_bandwidth.argtypes = [I, C.c_size_t, I, C.POINTER(D), C.POINTER(D), I]
     _LIB = L
     return L
 
@@ -234,7 +235,8 @@ def shape(kind, seed, nx, nz=None, length=1.0, eps=None, noise=0.0):
 
 
 def shape_ninit(ninit, nx, nz=None, length=1.0, eps=None):
-    """The reference's deterministic initial shape of ini field ninit (0, 1, 2, 4)."""
+    """The reference's deterministic initial shape of ini field ninit (0, 1, 2, 4, 5, 6, 8,
+    11, 50, 51)."""
     nz = nx if nz is None else nz
     if eps is None:
         eps = 2.0 * (2 * np.pi * length) / nx
@@ -545,3 +547,17 @@ def autotune(n, precision, steps=4, device=0):
     return {"n": r.n, "precision": r.precision, "steps": r.steps, "candidates": r.candidates,
             "chosen": r.chosen, "ms_per_step": [r.ms_per_step[i] for i in range(r.candidates)],
             "schedules": [schedule_describe(n, precision, i) for i in range(r.candidates)]}
+
+
+COPY_VARIANTS = ("1x16B", "4x16B", "1x16B nontemporal", "4x16B nontemporal")
+
+
+def copy_bandwidth(nbytes=1 << 30, repeats=20, device=0):
+    """Read + write GB/s of the library's own 16-byte copy kernels over an
+    nbytes buffer (vb_copy_bandwidth): the best and every variant's."""
+    best = C.c_double()
+    each = (C.c_double * len(COPY_VARIANTS))()
+    _check(lib().vb_copy_bandwidth(int(device), C.c_size_t(nbytes), int(repeats), C.byref(best), each,
+                                   len(COPY_VARIANTS)), "vb_copy_bandwidth")
+    return best.value, dict(zip(COPY_VARIANTS, list(each)))
+

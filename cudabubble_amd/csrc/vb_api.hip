@@ -44,7 +44,9 @@ int guard(F&& f) {
 
 const double kPI = 3.1415926535897932384626433832795;   // tool.h:7
 
-int validate(const vb_params* p) {
+}  // namespace
+
+int vb::validate_params(const vb_params* p) {
   if (!p) return VB_ERR_PARAM;
   if (p->nx <= 0 || p->nz <= 0) return VB_ERR_PARAM;
   if (p->nx % 2) return VB_ERR_ODD_NX;
@@ -57,6 +59,9 @@ int validate(const vb_params* p) {
   if (!(p->time_step > 0) || p->dt_double_every <= 0 || !(p->dt_cap > 0)) return VB_ERR_PARAM;
   return VB_OK;
 }
+
+namespace {
+int validate(const vb_params* p) { return vb::validate_params(p); }
 
 // splitmix64 (Steele, Lea, Flood 2014)
 struct Rng {
@@ -309,6 +314,40 @@ int vb_kernel_times(vb_solver* s, char* names, int name_len, int* counts, double
   return rc == VB_OK ? n : -rc;
 }
 
+}  // extern "C"
+
+namespace {
+// Name of the checkpoint field (fluidbase::output / input, fluidbase.cpp:259-275):
+// d<nx>x<ny>x<nz>.out, or for fields of at most 100000 bytes
+// d<nx>x<ny>x<nz>_<stamp>.out with stamp = roundx(t / TIME_STEP_OUTPUT)
+// (tool.cpp:23-26, TIME_STEP_OUTPUT = 1e-6 in stdafx.h:41).
+std::string field_file(const char* dir, const vb_params& p, double t) {
+  char fn[1024];
+  if ((long long)p.nx * p.nx * p.nz * 8 > 100000)
+    snprintf(fn, sizeof fn, "%s/d%dx%dx%d.out", dir, p.nx, p.nx, p.nz);
+  else
+    snprintf(fn, sizeof fn, "%s/d%dx%dx%d_%08d.out", dir, p.nx, p.nx, p.nz, int(floor(t / 1e-6 + 0.5)));
+  return fn;
+}
+
+// Sidecar of the field file: the solver's state spectrum, [nz][ny][nx/2+1]
+// complex doubles.  The reference format stores only the real field, and this
+// solver's state is the spectrum, which a transform round trip reproduces to
+// rounding only; with the sidecar a resumed run continues bit for bit.
+std::string spec_file(const std::string& field) { return field.substr(0, field.size() - 4) + ".spec"; }
+
+bool read_all(const std::string& fn, std::vector<double>& out) {
+  FILE* f = fopen(fn.c_str(), "rb");
+  if (!f) return false;
+  const size_t n = fread(out.data(), sizeof(double), out.size(), f);
+  const bool at_end = fgetc(f) == EOF;
+  fclose(f);
+  return n == out.size() && at_end;
+}
+}  // namespace
+
+extern "C" {
+
 // checkpoint files of fluidbase::output_info / output("d", ...) (fluidbase.cpp:224-265, 2776-2795)
 int vb_save(vb_solver* s, const char* dir) {
   if (!s || !s->impl || !dir) return VB_ERR_PARAM;
@@ -326,12 +365,22 @@ int vb_save(vb_solver* s, const char* dir) {
   const double info[7] = {t, dt, a, b, s->impl->info_extra[0], s->impl->info_extra[1], s->impl->info_extra[2]};
   const bool ok1 = fwrite(info, sizeof(double), 7, f) == 7;
   fclose(f);
-  snprintf(fn, sizeof fn, "%s/d%dx%dx%d.out", dir, p.nx, p.nx, p.nz);
-  f = fopen(fn, "wb");
+  const std::string ff = field_file(dir, p, t);
+  f = fopen(ff.c_str(), "wb");
   if (!f) return VB_ERR_IO;
   const bool ok2 = fwrite(phi.data(), sizeof(double), phi.size(), f) == phi.size();
   fclose(f);
-  return ok1 && ok2 ? VB_OK : VB_ERR_IO;
+  std::vector<double> spec((size_t)(p.nx / 2 + 1) * p.nx * p.nz * 2);
+  rc = guard([&] {
+    s->impl->get_spec(spec.data());
+    return (int)VB_OK;
+  });
+  if (rc != VB_OK) return rc;
+  f = fopen(spec_file(ff).c_str(), "wb");
+  if (!f) return VB_ERR_IO;
+  const bool ok3 = fwrite(spec.data(), sizeof(double), spec.size(), f) == spec.size();
+  fclose(f);
+  return ok1 && ok2 && ok3 ? VB_OK : VB_ERR_IO;
 }
 
 // fluidbase::load_info + reload (fluidbase.cpp:2797-2825, 2677-2686, 385-388)
@@ -350,14 +399,32 @@ int vb_load(vb_solver* s, const char* dir) {
   if (dt < 1e-13) dt = 1e-7;          // fluidbase.cpp:2818
   if (t == 0.0) t = 1e-13;            // fluidbase.cpp:2819
   std::vector<double> phi((size_t)p.nx * p.nx * p.nz);
-  snprintf(fn, sizeof fn, "%s/d%dx%dx%d.out", dir, p.nx, p.nx, p.nz);
-  f = fopen(fn, "rb");
+  const std::string ff = field_file(dir, p, t);
+  f = fopen(ff.c_str(), "rb");
   if (!f) return VB_ERR_IO;
   const size_t n = fread(phi.data(), sizeof(double), phi.size(), f);
   fclose(f);
   if (n != phi.size()) return VB_ERR_IO;
-  int rc = vb_set_phi(s, phi.data());
-  if (rc != VB_OK) return rc;
+  // the spectrum sidecar is used when it is the field's own: its inverse
+  // transform reproduces the field file bit for bit (a field written by
+  // another program, or edited, is taken as it is)
+  std::vector<double> spec((size_t)(p.nx / 2 + 1) * p.nx * p.nz * 2);
+  bool exact = false;
+  int rc = VB_OK;
+  if (read_all(spec_file(ff), spec)) {
+    std::vector<double> back(phi.size());
+    rc = guard([&] {
+      s->impl->set_spec(spec.data());
+      s->impl->get_phi(back.data());
+      return (int)VB_OK;
+    });
+    if (rc != VB_OK) return rc;
+    exact = memcmp(back.data(), phi.data(), sizeof(double) * phi.size()) == 0;
+  }
+  if (!exact) {
+    rc = vb_set_phi(s, phi.data());
+    if (rc != VB_OK) return rc;
+  }
   s->impl->info_extra[0] = info[4];
   s->impl->info_extra[1] = info[5];
   s->impl->info_extra[2] = info[6];

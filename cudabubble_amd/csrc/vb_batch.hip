@@ -154,10 +154,9 @@ int vb_batch_create(int n, const vb_params* params, const double* const* phi0, v
   *out = nullptr;
   for (int i = 0; i < n; ++i) {
     const vb_params& p = params[i];
-    if (p.nx <= 0 || p.nz <= 0 || !(p.eps > 0) || !(p.len > 0) || !phi0[i]) return VB_ERR_PARAM;
-    if (p.nx % 2) return VB_ERR_ODD_NX;
-    if (!vb::size_supported(p.nx) || !vb::size_supported(p.nz)) return VB_ERR_PARAM;
-    if (p.device != params[0].device) return VB_ERR_PARAM;
+    const int rc = vb::validate_params(&p);
+    if (rc != VB_OK) return rc;
+    if (!phi0[i] || p.device != params[0].device) return VB_ERR_PARAM;
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= params[0].device) return VB_ERR_NO_DEVICE;

@@ -26,6 +26,11 @@ struct HipError {
 
 void set_last_error(const std::string& msg);
 
+// VB_OK, or the status vb_create returns for these parameters (grid, lengths,
+// scheme, precision, table mode, time-step settings); every entry point that
+// takes a vb_params checks it
+int validate_params(const vb_params* p);
+
 // ---- host tables (vb_lut.cpp, compiled without FMA contraction) -----------
 // L^ of Calc::init_fEIF (Calc.cu:599-600) for c = -m (2 pi / L)^2.
 double lhat(double m, double kfac2, double eps, double kappa, double D);

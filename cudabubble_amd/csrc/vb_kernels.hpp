@@ -197,8 +197,7 @@ struct XaArgs {
 // batched launch over the solves of a size group) from a per-solve table in
 // device memory indexed by blockIdx.z.
 // k_xa: the x kernel of the Euler paths (WITHF: v, f and the x-forward of f
-// as well) and of the all-FP64 integrals of mixed mode; partial sums and the
-// x-forward of tmp1, two rows per lane.
+// as well); partial sums and the x-forward of tmp1, two rows per lane.
 template <typename T, int N, bool WITHF, bool TAB = false>
 __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
@@ -295,7 +294,10 @@ template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() 
 // where tmp1 is O(1), and the rounding of a shared transform scales with the
 // larger part: s goes in as eps^3 s, of tmp1's size, and its spectrum is
 // scaled back at the store.
-template <typename T, int N, bool TAB = false, int XP = xrplan<T, N>()>
+// SUMS: the partial sums only (no forward transform, nothing stored): the
+// integrals V, A, E_b of a field, as in FP32-mixed mode's FP64 energy of the
+// state (T = double).
+template <typename T, int N, bool TAB = false, int XP = xrplan<T, N>(), bool SUMS = false>
 __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(), LL = row_lds_len<N>(),
@@ -328,6 +330,10 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
     const T sv = T(D) * t1 + (T(3) * v * v - T(D + 1)) * (T(eps) * f);  // eps^3 s, Calc.cu:282
     if (valid) { acc[0] += double(v); acc[1] += double(w) * double(w); acc[2] += double(f) * double(f); }
     x[e] = mk<T>(t1, sv);
+  }
+  if constexpr (SUMS) {
+    block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
+    return;
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
   unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));

@@ -17,12 +17,14 @@ namespace vb {
 // then issues one launch per recorded position for all solves of a size
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
-  OP_XA, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_Y_FWD_GM, OP_ZLAP, OP_ZPREP,
+  OP_XA_RS, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_Y_FWD_GM, OP_ZLAP, OP_ZPREP,
   OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET, OP_ZS44P
 };
-// x-kernel outputs: tmp1 only, tmp1 and f (Euler), tmp1 and s (spectral N),
-// the partial sums of v and (v^2-1)^2 only (from P alone: k_xsums)
-enum XaMode { XA_PLAIN = 0, XA_WITHF = 1, XA_WITHS = 2, XA_SUMS = 3 };
+// x-kernel outputs: the partial sums of v, (v^2-1)^2 and f^2 only (k_xrow
+// without its forward transform: the integrals of a field), tmp1 and f
+// (Euler), tmp1 and s (spectral N), the partial sums of v and (v^2-1)^2 only
+// (from P alone: k_xsums)
+enum XaMode { XA_ROWSUMS = 0, XA_WITHF = 1, XA_WITHS = 2, XA_SUMS = 3 };
 struct RecLaunch {
   // issues the table form of the recorded kernel over `lanes` solves
   void (*launch_tab)(int op, const void* tab, dim3 grid, dim3 block, int lanes, hipStream_t s);

@@ -264,9 +264,8 @@ int vb_server_create(const vb_server_config* cfg, vb_server** out) {
 
 int vb_server_submit(vb_server* s, long long id, const vb_params* p, const double* phi0) {
   if (!s || !p || !phi0) return VB_ERR_PARAM;
-  if (p->nx <= 0 || p->nz <= 0 || !(p->eps > 0) || !(p->len > 0)) return VB_ERR_PARAM;
-  if (p->nx % 2) return VB_ERR_ODD_NX;
-  if (!vb::size_supported(p->nx) || !vb::size_supported(p->nz)) return VB_ERR_PARAM;
+  const int rc = vb::validate_params(p);
+  if (rc != VB_OK) return rc;
   if (p->fixed_time_step > 0 || p->scheme == VB_EULER) return VB_ERR_PARAM;   // adaptive ETD only
   if (p->device != s->cfg.device) return VB_ERR_PARAM;
   Request q;

@@ -18,7 +18,10 @@ extern "C" {
 
 int vb_shape_ninit(int ninit, int nx, int ny, int nz, double len, double eps, double* phi) {
   if (!phi || nx <= 0 || ny <= 0 || nz <= 0 || !(len > 0) || !(eps > 0)) return VB_ERR_PARAM;
-  if (ninit != 0 && ninit != 1 && ninit != 2 && ninit != 4) return VB_ERR_PARAM;
+  switch (ninit) {
+    case 0: case 1: case 2: case 4: case 5: case 6: case 8: case 11: case 50: case 51: break;
+    default: return VB_ERR_PARAM;
+  }
   const double L = len * 2.0 * kPI;   // xlen = ylen = zlen (Calc.cu:48-49)
   const double s2 = sqrt(2.0);
   std::vector<double> xx(nx), yy(ny), zz(nz);
@@ -45,6 +48,47 @@ int vb_shape_ninit(int ninit, int nx, int ny, int nz, double len, double eps, do
           // spheroid with z semi-axis 0.7 of the others, half-width profile (fluidbase.cpp:1862-1877)
           const double r = sqrt((x * x + y * y) + z * z / 0.49);
           val = tanh((0.7 * L / 2.0 - r) / (0.5 * s2 * eps));
+        } else if (ninit == 5 || ninit == 6 || ninit == 11) {
+          // oblate spheroids: x, y semi-axis a, z semi-axis a / q with q = 2, 5, 3
+          // and a = 0.83, 0.8, 0.8 L/2, half-width profile (init_ellipse2,
+          // init_ellipse3, init_ellipse4, fluidbase.cpp:1880-1932)
+          const double q = ninit == 5 ? 2.0 : ninit == 6 ? 5.0 : 3.0;
+          const double a = ninit == 5 ? 0.83 : 0.8;
+          const double r = sqrt((x * x + y * y) / (q * q) + z * z) * q;
+          val = tanh((a * L / 2.0 - r) / (0.5 * s2 * eps));
+        } else if (ninit == 8) {
+          // torus about the y axis: centre circle of radius 0.5 L/2 in the x-z
+          // plane, tube radius 0.25 L/2 (init_vtorus, fluidbase.cpp:2121-2143)
+          const double r = sqrt(x * x + z * z);
+          if (r == 0) {
+            val = tanh(-0.5 * L / 2.0 / (s2 * eps));
+          } else {
+            double th = acos(x / r);
+            if (z < 0) th = 2 * kPI - th;
+            const double cx = x - 0.5 * L / 2.0 * cos(th), cz = z - 0.5 * L / 2.0 * sin(th);
+            const double d = 0.25 * L / 2.0 - sqrt(cx * cx + cz * cz + y * y);
+            val = tanh(d / (s2 * eps));
+          }
+        } else if (ninit == 50) {
+          // "tian": a 3 x 3 x 3 frame of axis-parallel rods of radius r = L/12
+          // through the planes 0 and +-4r, cut at |x|, |y|, |z| = 5r; a sharp
+          // +-1 field (init_tian, fluidbase.cpp:1825-1858)
+          const double r = L / 12, rr = r * r;
+          const double X[3] = {x, x - 4 * r, x + 4 * r}, Y[3] = {y, y - 4 * r, y + 4 * r},
+                       Z[3] = {z, z - 4 * r, z + 4 * r};
+          bool in = false;
+          // the reference lists, for each pair of axes, the rods at all nine offset
+          // pairs in {0, 4r, -4r}^2
+          for (int a = 0; a < 3 && !in; ++a)
+            for (int b = 0; b < 3 && !in; ++b)
+              in = X[a] * X[a] + Y[b] * Y[b] < rr || X[a] * X[a] + Z[b] * Z[b] < rr || Y[a] * Y[a] + Z[b] * Z[b] < rr;
+          if (fabs(x) > 5 * r || fabs(y) > 5 * r || fabs(z) > 5 * r) in = false;
+          val = in ? 1.0 : -1.0;
+        } else if (ninit == 51) {
+          // the gyroid cos x sin y + cos y sin z + cos z sin x > 0, a sharp +-1
+          // field (init_Gyroid, fluidbase.cpp:2255-2270)
+          const double g = cos(x) * sin(y) + cos(y) * sin(z) + cos(z) * sin(x);
+          val = g > 0 ? 1.0 : -1.0;
         } else {
           // torus whose tube radius varies with the angle (fluidbase.cpp:2068-2090)
           const double r = sqrt(x * x + y * y);

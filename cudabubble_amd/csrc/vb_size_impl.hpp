@@ -53,9 +53,9 @@ struct Launch {
     // k_xrow: one row per lane, over the same rows per workgroup; k_xsums:
     // kXsLoop row groups per workgroup
     const dim3 gr(mode == XA_SUMS ? (xgrid(a.npairs) + kXsLoop - 1) / kXsLoop : xgrid(a.npairs)),
-        bl(mode == XA_WITHS ? xrthreads<T, N, XR>() : XWG);
+        bl(mode == XA_WITHS || mode == XA_ROWSUMS ? xrthreads<T, N, XR>() : XWG);
     if (g_recorder) {
-      const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA;
+      const int op = mode == XA_WITHF ? OP_XA_F : mode == XA_WITHS ? OP_XA_S : mode == XA_SUMS ? OP_XA_SUMS : OP_XA_RS;
       return record_launch(&launch_tab, op, a, gr, bl);
     }
     if (mode == XA_WITHF)
@@ -65,7 +65,7 @@ struct Launch {
     else if (mode == XA_SUMS)
       hipLaunchKernelGGL((k_xsums<T, N>), gr, bl, 0, s, a, nullptr);
     else
-      hipLaunchKernelGGL((k_xa<T, N, false>), gr, bl, 0, s, a, nullptr);
+      hipLaunchKernelGGL((k_xrow<T, N, false, XR, true>), gr, bl, 0, s, a, nullptr);
   }
   static void xe(const XeArgs<T>& a, hipStream_t s) {
     hipLaunchKernelGGL((k_xe<T, N>), dim3(xgrid(a.npairs)), dim3(XWG), 0, s, a);
@@ -145,7 +145,9 @@ struct Launch {
   static void launch_tab(int op, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
     gr.z = unsigned(lanes);
     switch (op) {
-      case OP_XA: hipLaunchKernelGGL((k_xa<T, N, false, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
+      case OP_XA_RS:
+        hipLaunchKernelGGL((k_xrow<T, N, true, XR, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);
+        break;
       case OP_XA_F: hipLaunchKernelGGL((k_xa<T, N, true, true>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab); break;
       case OP_XA_S:
         hipLaunchKernelGGL((k_xrow<T, N, true, XR>), gr, bl, 0, s, XaArgs<T>{}, (const XaArgs<T>*)tab);

@@ -166,10 +166,13 @@ struct Solver final : SolverBase {
   DBuf<FinScratch> fin_scratch;
   DBuf<cpx<T>> twx, twz;
   DBuf<cpx<double>> twxd, twzd;
-  // FP64 evaluation of the integrals of the state (mixed precision only)
-  DBuf<cpx<double>> P64, Q64, T64;
+  // FP32-mixed only: FP64 integrals of the state (FP64 transforms of the
+  // FP64 state), for the energy-decrease test of the adaptive driver and for
+  // vb_query
+  DBuf<cpx<double>> P64, Q64;
   DBuf<double> xpart64, zpart64, dc64;
   DBuf<Scal> scal64;
+  int nxb64 = 0, nzb64 = 0;
   Scal* hs = nullptr;   // pinned host copy
   DBuf<double> rpart;   // block sums of the Euler residual
   int nxb = 0, nzb = 0;
@@ -810,12 +813,22 @@ struct Solver final : SolverBase {
   double cur_E = 0.0, att_eng0 = 0.0;
   int att_keep = 0;
 
+  // the energy-decrease test compares FP64 energies in both precisions: in
+  // FP32-mixed mode they come from FP64 transforms of the FP64 state
+  static constexpr bool kMixed = sizeof(T) == 4;
+
   void adaptive_issue(vb_step_report& r) override {
     if (!started) start();
+    if (kMixed) need64();   // allocations stay outside graph capture
     if (!iter_open) {
       if (!cur_E_known) {
-        ensure_prefixed();
-        read_scal();
+        if (kMixed) {
+          energy64();
+          read_scal64();
+        } else {
+          ensure_prefixed();
+          read_scal();
+        }
         cur_E = hs->E;
         cur_E_known = true;
       }
@@ -841,16 +854,19 @@ struct Solver final : SolverBase {
     } else {
       attempt_body();
     }
-    VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipMemcpyAsync(hs, kMixed ? scal64.p : scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipEventRecord(att_ev, st));
     att_pending = true;
   }
 
-  // the launches of one adaptive attempt: evaluation and candidate energy
+  // the launches of one adaptive attempt: the evaluation, the prefix of the
+  // candidate (its integrals; the first stage of the next attempt if it is
+  // accepted) and, in FP32-mixed mode, its FP64 energy
   void attempt_body() {
     const int out = eval(time_step);
     cur = out;
     prefix(FIN_PLAIN, false);
+    if (kMixed) energy64();
   }
 
   bool attempt_ready() override {
@@ -915,6 +931,10 @@ struct Solver final : SolverBase {
 
   void read_scal() {
     VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+  }
+  void read_scal64() {
+    VB_HIP(hipMemcpyAsync(hs, scal64.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
   }
 
@@ -1072,45 +1092,46 @@ struct Solver final : SolverBase {
     if (rep) *rep = r;
   }
 
-  // Integrals of the state evaluated entirely in FP64 (mixed precision):
-  // FP64 z, y and x transforms of the FP64 state, FP64 f and reductions.
-  void integrals64() {
-    const size_t nc = (size_t)g.ncp;
-    if (!P64.p) {
-      P64.alloc(nc);
-      Q64.alloc(nc);
-      T64.alloc(nc);
-      dc64.alloc(1);
-      scal64.alloc(1);
+  void need64() {
+    if (P64.p) return;
+    P64.alloc((size_t)g.ncp);
+    Q64.alloc((size_t)g.ncp);
+    dc64.alloc(1);
+    scal64.alloc(1);
+    nxb64 = int((npairs + oxd->xlanes - 1) / oxd->xlanes);
+    nzb64 = int(zgroups(g, ozd->ytile));
+    xpart64.alloc((size_t)nxb64 * 3);
+    zpart64.alloc((size_t)nzb64);
+  }
+
+  // FP32-mixed: V, A, E_b and E of uh[cur] evaluated entirely in FP64 into
+  // scal64 (launches only): FP64 z and y inverse transforms of the FP64 state
+  // and of its Laplacian, the FP64 x inverse with f and the partial sums
+  // (k_xrow without its forward transform), the finalize kernel under the
+  // targets of scal.  need64() first.
+  void energy64() {
+    const cpx<double>* src = uh[cur].p;
+    if (ozd->ytile != oz->ytile) {   // the FP64 z kernels have their own column tile
+      L("regroup", [&] { relayout(uh[cur].p, oz->ytile, S.p, ozd->ytile); });
+      src = S.p;
     }
-    const int nxb64 = int((npairs + oxd->xlanes - 1) / oxd->xlanes);
-    const int nzb64 = int(zgroups(g, ozd->ytile));
-    if (xpart64.n < (size_t)nxb64 * 3) xpart64.alloc((size_t)nxb64 * 3);
-    if (zpart64.n < (size_t)nzb64) zpart64.alloc((size_t)nzb64);
-    // the FP64 z kernels have their own column tile: regroup the state for them
-    L("regroup", [&] { relayout(uh[cur].p, oz->ytile, S.p, ozd->ytile); });
-    L("zprep64", [&] { ozd->zprep(S.p, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
+    L("zprep64", [&] { ozd->zprep(src, P64.p, Q64.p, zpart64.p, dc64.p, g, twzd.p, st); });
     L("y_inv2_64", [&] { oxd->ypass(P64.p, Q64.p, 2, +1, g, twxd.p, nullptr, st); });
-    XaArgs<double> a;
+    XaArgs<double> a{};
     a.P = P64.p;
     a.Q = Q64.p;
-    a.v = nullptr;   // the integrals need no real fields stored
-    a.f = nullptr;
-    a.Tt = T64.p;
-    a.W = nullptr;
     a.part = xpart64.p;
     a.tw = twxd.p;
     a.npairs = npairs;
     a.eps = p.eps;
     a.kappa = p.kappa;
+    a.D = p.D;
     a.inv_n = 1.0 / double(g.nr);
     a.rl = Rows{g.ny, g.pitch, g.pp};
-    L("xa64", [&] { oxd->xa(a, XA_PLAIN, st); });
-    VB_HIP(hipMemcpyAsync(scal64.p, scal.p, sizeof(Scal), hipMemcpyDeviceToDevice, st));
-    L("finalize", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
+    L("xsums64", [&] { oxd->xa(a, XA_ROWSUMS, st); });
+    launch_keep(OP_KEEP, scal.p, scal64.p, ph, st);   // the targets
+    L("finalize64", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
     check_launch();
-    VB_HIP(hipMemcpyAsync(hs, scal64.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
-    VB_HIP(hipStreamSynchronize(st));
   }
 
   void query(double* E, double* V, double* A, double* Eb) override {
@@ -1119,7 +1140,9 @@ struct Solver final : SolverBase {
       read_scal();
     } else {
       if (targets_drift()) sync_targets_from_device();
-      integrals64();
+      need64();
+      energy64();
+      read_scal64();
     }
     if (E) *E = hs->E;
     if (V) *V = hs->V;
@@ -1169,6 +1192,35 @@ struct Solver final : SolverBase {
     L("x_r2c", [&] { oxd->xr2c_d(phid.p, S.p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
     L("y_fwd", [&] { oxd->ypass(S.p, nullptr, 1, -1, g, twxd.p, nullptr, st); });
     L("z_fwd", [&] { ozd->zpass(S.p, -1, g, twzd.p, st); });
+    L("to_groups", [&] { relayout(S.p, 0, uh[cur].p, oz->ytile); });
+    check_launch();
+    VB_HIP(hipStreamSynchronize(st));
+    prepped = prefixed = prefixed_f = false;
+  }
+
+  // the state spectrum in the reference layout [nz][ny][nx/2+1] (the
+  // solver's rows are padded to g.pitch, planes to g.pp)
+  void get_spec(double* host) override {
+    L("to_rows", [&] { relayout(uh[cur].p, oz->ytile, S.p, 0); });
+    check_launch();
+    std::vector<cpx<double>> h((size_t)g.ncp);
+    VB_HIP(hipMemcpyAsync(h.data(), S.p, sizeof(cpx<double>) * h.size(), hipMemcpyDeviceToHost, st));
+    VB_HIP(hipStreamSynchronize(st));
+    const size_t nh = (size_t)g.nx / 2 + 1;
+    for (size_t z = 0; z < (size_t)g.nz; ++z)
+      for (size_t y = 0; y < (size_t)g.ny; ++y)
+        memcpy(host + 2 * ((z * g.ny + y) * nh), &h[z * g.pp + y * g.pitch], sizeof(cpx<double>) * nh);
+  }
+
+  void set_spec(const double* host) override {
+    cur_E_known = false;
+    iter_open = false;
+    const size_t nh = (size_t)g.nx / 2 + 1;
+    std::vector<cpx<double>> h((size_t)g.ncp, cpx<double>{0.0, 0.0});
+    for (size_t z = 0; z < (size_t)g.nz; ++z)
+      for (size_t y = 0; y < (size_t)g.ny; ++y)
+        memcpy(&h[z * g.pp + y * g.pitch], host + 2 * ((z * g.ny + y) * nh), sizeof(cpx<double>) * nh);
+    VB_HIP(hipMemcpyAsync(S.p, h.data(), sizeof(cpx<double>) * h.size(), hipMemcpyHostToDevice, st));
     L("to_groups", [&] { relayout(S.p, 0, uh[cur].p, oz->ytile); });
     check_launch();
     VB_HIP(hipStreamSynchronize(st));
@@ -1229,7 +1281,7 @@ size_t solver_footprint(const vb_params& p) {
   const bool euler = p.scheme == VB_EULER || p.shape_adjust;
   const size_t work = 10 + (euler ? 1 : 0);   // P, Q, Tt, Ts, Tt1, Ts1, pre, Na, Nb, Vh (+ W)
   size_t bytes = 3 * nc * 16 + work * nc * cs + (euler ? 2 * nr * rs : 0) + nr * 8;
-  if (mixed) bytes += 3 * nc * 16;   // FP64 query buffers
+  if (mixed) bytes += 2 * nc * 16;   // FP64 integrals of the state (P64, Q64)
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
   return bytes + 2 * mm * 48 + ((size_t)4 << 20);
 }

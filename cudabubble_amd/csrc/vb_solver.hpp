@@ -34,6 +34,10 @@ struct SolverBase {
   virtual void set_state(double alpha, double beta, double dt, double t, double alpha0, double beta0) = 0;
   virtual void get_phi(double* host) = 0;
   virtual void set_phi(const double* host) = 0;
+  // the state spectrum u^ as [nz][ny][nx/2+1] complex doubles (the real
+  // field's forward transform, normalised as Calc::r2c_3d_f)
+  virtual void get_spec(double* host) = 0;
+  virtual void set_spec(const double* host) = 0;
   virtual void evaluate(double h) = 0;
   virtual double residual() = 0;
   virtual hipStream_t stream() = 0;

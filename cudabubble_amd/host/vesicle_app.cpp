@@ -3,6 +3,7 @@
 
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -130,12 +131,8 @@ void belastic::output_field(const double* phi) {
     }
     fclose(f);
   }
-  if ((size_t)nx * ny * nz * 8 > 100000) return;   // large grids: d<dims>.out of the checkpoint
-  snprintf(fn, sizeof fn, "%sd%dx%dx%d_%08d.out", folder_.c_str(), nx, ny, nz, stamp);
-  if (FILE* f = fopen(fn, "w+b")) {
-    fwrite(phi, sizeof(double), (size_t)nx * ny * nz, f);
-    fclose(f);
-  }
+  // the raw field itself is the checkpoint's (vb_save: d<dims>.out, or
+  // d<dims>_<stamp>.out for fields of at most 100000 bytes)
 }
 
 double belastic::energy(double* V, double* A, double* Eb) {
@@ -193,15 +190,32 @@ bool belastic::load_initial() {
   if (!resume) {
     output("Starting a new experiment.... start time is 0.\n");
     char fn[1024];
-    if (ninit_ == 0 || ninit_ == 1 || ninit_ == 2 || ninit_ == 4) {
+    // a field file that cannot be read leaves zeros, as read_array does (tool.cpp:44-53)
+    auto load_or_zero = [&](const char* path) {
+      if (vb_read_field(path, p_.nx, p_.nx, p_.nz, phi.data()) != VB_OK) {
+        char msg[1200];
+        snprintf(msg, sizeof msg, "Can not load file %s! set as zeros\n", path);
+        output(msg);
+        std::fill(phi.begin(), phi.end(), 0.0);
+      }
+    };
+    const int analytic[] = {0, 1, 2, 4, 5, 6, 8, 11, 50, 51};
+    if (std::find(std::begin(analytic), std::end(analytic), ninit_) != std::end(analytic)) {
       check(vb_shape_ninit(ninit_, p_.nx, p_.nx, p_.nz, p_.len, p_.eps, phi.data()), "vb_shape_ninit");
-    } else if (ninit_ == 7) {   // the field saved in the folder (fluidbase.cpp:560-562)
-      snprintf(fn, sizeof fn, "%sd%dx%dx%d.out", folder_.c_str(), p_.nx, p_.nx, p_.nz);
-      check(vb_read_field(fn, p_.nx, p_.nx, p_.nz, phi.data()), fn);
+    } else if (ninit_ == 7) {   // the field saved in the folder (fluidbase.cpp:560-562, 268-275), t = 0
+      if ((size_t)p_.nx * p_.nx * p_.nz * 8 > 100000)
+        snprintf(fn, sizeof fn, "%sd%dx%dx%d.out", folder_.c_str(), p_.nx, p_.nx, p_.nz);
+      else
+        snprintf(fn, sizeof fn, "%sd%dx%dx%d_%08d.out", folder_.c_str(), p_.nx, p_.nx, p_.nz, 0);
+      load_or_zero(fn);
     } else if (ninit_ == 35) {  // the dimpled disk of the results folder (fluidbase.cpp:2232-2238)
       snprintf(fn, sizeof fn, "%sdiskflat%dx%dx%d.out", result_path_.c_str(), p_.nx, p_.nx, p_.nz);
       if (!exists(fn)) snprintf(fn, sizeof fn, "%sdisk%dx%dx%d.out", result_path_.c_str(), p_.nx + 1, p_.nx + 1, p_.nz);
-      check(vb_read_field(fn, p_.nx, p_.nx, p_.nz, phi.data()), fn);
+      load_or_zero(fn);
+    } else if (ninit_ == 43) {  // the drop of the results folder (fluidbase.cpp:2217-2228)
+      snprintf(fn, sizeof fn, "%sdrop%dx%dx%d.out", result_path_.c_str(), p_.nx, p_.nx, p_.nz);
+      if (!exists(fn)) snprintf(fn, sizeof fn, "%sdrop%dx%dx%d.out", result_path_.c_str(), p_.nx + 1, p_.nx + 1, p_.nz);
+      load_or_zero(fn);
     } else {
       throw std::runtime_error("initial shape ninit = " + std::to_string(ninit_) + " is not supported");
     }

@@ -118,7 +118,12 @@ int vb_residual(vb_solver* s, double* err);
  * (belastic.cpp:519-552, 491-517) with step h */
 int vb_evaluate(vb_solver* s, double h);
 /* checkpoint: <dir>/<nx>x<ny>x<nz>.info (7 doubles, fluidbase.cpp:2776-2795)
- * and <dir>/d<nx>x<ny>x<nz>.out (raw doubles, fluidbase.cpp:224-265) */
+ * and the raw field <dir>/d<nx>x<ny>x<nz>.out, or d<nx>x<ny>x<nz>_<stamp>.out
+ * for fields of at most 100000 bytes, stamp = round(t / 1e-6) as %08d
+ * (fluidbase.cpp:259-275).  vb_save also writes the state spectrum beside
+ * the field (same name, .spec: [nz][ny][nx/2+1] complex doubles); vb_load
+ * uses it when its inverse transform is the field file bit for bit, so a
+ * resumed run continues exactly, and otherwise loads the field. */
 int vb_save(vb_solver* s, const char* dir);
 int vb_load(vb_solver* s, const char* dir);
 /* device stream the solver launches on (a hipStream_t) */
@@ -177,8 +182,12 @@ int vb_shape(int kind, unsigned long long seed, int nx, int nz, double len, doub
              double noise, double* phi_out, double* shape_params /* optional [8] */);
 
 /* The reference's deterministic initial shapes by ini field ninit
- * (fluidbase.cpp:539-552): 0 sphere (:1080-1099), 1 two spheres (:1197-1215),
- * 2 spheroid (:1862-1877), 4 torus of varying tube radius (:2068-2090). */
+ * (fluidbase.cpp:539-699): 0 sphere (:1080-1099), 1 two spheres (:1197-1215),
+ * 2 spheroid (:1862-1877), 4 torus of varying tube radius (:2068-2090),
+ * 5, 6, 11 oblate spheroids 2:1, 5:1, 3:1 (:1880-1932), 8 torus about the y
+ * axis (:2121-2143), 50 the "tian" rod frame (:1825-1858) and 51 the gyroid
+ * (:2255-2270), both sharp +-1 fields.  All axes share the box length
+ * 2 pi len.  The file-based shapes (7, 35, 43) are read with vb_read_field. */
 int vb_shape_ninit(int ninit, int nx, int ny, int nz, double len, double eps, double* phi_out);
 /* A raw field file as write_array stores it (tool.cpp:28-42): nx*ny*nz
  * doubles, or (nx+1)*(ny+1)*nz doubles whose last x and y planes repeat the
@@ -283,6 +292,15 @@ int vb_autotune(int n, int precision, int steps, int device, vb_tune_report* rep
  * (Calc.cu:453-475); H = L^ of Calc::init_fEIF (Calc.cu:599-600). */
 int vb_lut(int scheme, int lut_mode, double h, double eps, double len, double kappa, double D, int mmax,
            double* out);
+
+/* ---- measurement -------------------------------------------------------------
+ * No reference counterpart.  The streaming ceiling of the device: copies of a
+ * `bytes` buffer into a second one with 16-byte loads and stores (one or four
+ * per thread, plain or nontemporal), `repeats` times each, timed with HIP
+ * events; read + write bytes per second of the fastest variant in *best_GBps,
+ * and of each variant in variant_GBps[0..cap) (order: 1x16B, 4x16B,
+ * 1x16B nontemporal, 4x16B nontemporal).  Allocates and frees its buffers. */
+int vb_copy_bandwidth(int device, size_t bytes, int repeats, double* best_GBps, double* variant_GBps, int cap);
 
 const char* vb_version(void);
 

@@ -225,3 +225,28 @@ def test_autotune_arguments_and_no_device(vb):
         vb.autotune(32, vb.FP32_MIXED, steps=1)
     assert e.value.status == vb.VB_ERR_NO_DEVICE
     assert vb.schedule_get(32, vb.FP32_MIXED) == saved   # a failed tuning selects nothing
+
+
+@pytest.mark.parametrize("field,value,status", [
+    ("dt_double_every", 0, "VB_ERR_PARAM"), ("time_step", 0.0, "VB_ERR_PARAM"), ("dt_cap", -1.0, "VB_ERR_PARAM"),
+    ("scheme", 3, "VB_ERR_PARAM"), ("precision", 2, "VB_ERR_PARAM"), ("phi_lut_mode", 5, "VB_ERR_PARAM"),
+    ("eps", 0.0, "VB_ERR_PARAM"), ("nx", 33, "VB_ERR_ODD_NX"), ("nx", 100, "VB_ERR_PARAM")])
+def test_invalid_params_rejected_by_every_entry_point(vb, field, value, status):
+    # vb_create, vb_footprint and vb_batch_create share one check, made before
+    # any device work (a zero doubling period would otherwise divide by zero in
+    # the adaptive driver, belastic.cpp:644)
+    import ctypes as C
+    p = vb.params_default(nx=32)
+    setattr(p, field, value)
+    code = getattr(vb, status)
+    phi = np.zeros((p.nz, p.nx, p.nx))
+    with pytest.raises(vb.VbError) as e:
+        vb.Solver(p, phi)
+    assert e.value.status == code
+    assert vb.footprint(p) == 0
+    arr = (vb.Params * 1)(p)
+    ptrs = (C.c_void_p * 1)(phi.ctypes.data)
+    h = C.c_void_p()
+    assert vb.lib().vb_batch_create(1, arr, ptrs, C.byref(h)) == code
+    zero = vb.Params()
+    assert vb.lib().vb_batch_create(1, (vb.Params * 1)(zero), ptrs, C.byref(h)) == vb.VB_ERR_PARAM

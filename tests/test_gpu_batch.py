@@ -153,3 +153,30 @@ def test_batch_large_sizes_match_solo(vb, gpu):
         ref_phi, _, ref_q, _ = solo(vb, p, phi, [4], graphs=0)
         assert np.array_equal(got[i][0], ref_phi), i
         assert got[i][1] == ref_q, i
+
+
+def test_batched_mix_48_solves(vb, gpu):
+    # the batched configuration as stated: 16 x 32^3, 16 x 64^3, 12 x 128^3 and
+    # 4 x 192^3 FP32-mixed ETDRK4 solves with the penalty drift, 300 steps each in
+    # one batch; every solve ends without a driver exit, and one solve of every
+    # size group matches its own solo run bit for bit
+    from cudabubble_amd.workloads import batch_mix
+    ps, phis = batch_mix(vb)
+    assert len(ps) == 48
+    flush_print("48-solve mix: building")
+    b = vb.Batch(ps, phis)
+    reps = b.step(300)
+    assert all(r["status"] == 0 and r["steps"] == 300 and r["evaluations"] == 600 for r in reps), \
+        [(i, r) for i, r in enumerate(reps) if r["status"] != 0 or r["steps"] != 300][:4]
+    first = {}
+    for i, p in enumerate(ps):
+        first.setdefault(p.nx, i)
+    got = {n: (b.get_phi(i), b.query(i)) for n, i in first.items()}
+    b.close()
+    for n, i in sorted(first.items()):
+        flush_print("48-solve mix: solo run of solve", i, "n =", n)
+        s = vb.Solver(ps[i], phis[i])
+        assert s.step(300)["status"] == 0
+        assert np.array_equal(s.get_phi(), got[n][0]), n
+        assert s.query() == got[n][1], n
+        s.close()
