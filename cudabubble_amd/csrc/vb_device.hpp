@@ -39,6 +39,9 @@ struct Scal {
   double alpha, beta;          // current targets
   double alpha0, beta0;        // targets at t = 0 (drift reference)
   double sum_v, sum_w, sum_f2, grad2;
+  // FP32-mixed stage fields: the means of tmp1 and of eps^3 s, summed in FP64
+  // from the x kernel's values (the k = 0 coefficients of their spectra)
+  double dc_t1, dc_s;
   int nonfinite, pad;
 };
 
@@ -94,7 +97,7 @@ __device__ __forceinline__ void apply_targets(Scal& c, const Phys& ph) {
 // left zeroed by every evaluation.
 constexpr int kFinShares = 16;
 struct FinScratch {
-  double part[4 * kFinShares];
+  double part[6 * kFinShares];
   unsigned ticket, pad;
 };
 
@@ -105,6 +108,7 @@ struct FinArgs {
   const double* xpart;
   const double* zpart;
   const double* dc;   // k = 0 coefficient of the field (its mean)
+  const double* dcpart;   // FP32-mixed: the x kernel's sums of tmp1 and eps^3 s (2 per block), or nullptr
   Scal* sc;
   FinScratch* fs;
   int nxb, nzb, mode, pad;
@@ -118,7 +122,8 @@ struct FinArgs {
 // them in workgroup order, so the result does not depend on which one
 // finishes last.  Thread 0 of the final workgroup writes V, A, E_b, E and the
 // multipliers (Calc.cu:191-263, phasefield.cpp:283-325, belastic.cpp:68-86
-// and 473-479).  red holds 4 * (NT / 64) doubles and tot 5 doubles of LDS.
+// and 473-479), and with dcpart the means of tmp1 and eps^3 s.  red holds
+// 6 * (NT / 64) doubles and tot 7 doubles of LDS.
 // UN accumulator sets per thread keep UN loads in flight (fewer registers in
 // kernels that carry this as a side branch).
 template <int NT, int UN = 4>
@@ -140,29 +145,36 @@ __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, dou
     }
   }
   for (int i = z0 + threadIdx.x; i < z1; i += NT) u[0][3] += a.zpart[i];
-  double s[4];
+  double s[6];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     s[q] = u[0][q];
 #pragma unroll
     for (int k = 1; k < UN; ++k) s[q] += u[k][q];
   }
+  s[4] = s[5] = 0.0;
+  if (a.dcpart) {
+    for (int i = x0 + threadIdx.x; i < x1; i += NT) {
+      s[4] += a.dcpart[2 * i];
+      s[5] += a.dcpart[2 * i + 1];
+    }
+  }
   if (nb == 1) {
-    block_sum_store<4, NT>(s, red, tot);
+    block_sum_store<6, NT>(s, red, tot);
     __syncthreads();
   } else {
-    unsigned* last = reinterpret_cast<unsigned*>(tot + 4);
-    block_sum_store<4, NT>(s, red, a.fs->part + 4 * b);
+    unsigned* last = reinterpret_cast<unsigned*>(tot + 6);
+    block_sum_store<6, NT>(s, red, a.fs->part + 6 * b);
     __threadfence();   // publish the share before taking a ticket
     __syncthreads();
     if (threadIdx.x == 0) *last = atomicAdd(&a.fs->ticket, 1u) == unsigned(nb - 1) ? 1u : 0u;
     __syncthreads();
     if (!*last) return;
     __threadfence();
-    if (threadIdx.x < 4) {
+    if (threadIdx.x < 6) {
       const volatile double* sh = a.fs->part;
       double t = 0.0;
-      for (int k = 0; k < nb; ++k) t += sh[4 * k + threadIdx.x];
+      for (int k = 0; k < nb; ++k) t += sh[6 * k + threadIdx.x];
       tot[threadIdx.x] = t;
     }
     if (threadIdx.x == 0) a.fs->ticket = 0u;
@@ -192,6 +204,10 @@ __device__ __forceinline__ void finalize_body(const FinArgs& a, double* red, dou
     c.sum_w = tot[1];
     c.sum_f2 = tot[2];
     c.grad2 = tot[3];
+    if (a.dcpart) {
+      c.dc_t1 = tot[4] / N;
+      c.dc_s = tot[5] / N;
+    }
     c.nonfinite = !(isfinite(V) && isfinite(A) && isfinite(Eb)) ? 1 : 0;
     apply_targets(c, a.ph);
     *a.sc = c;

@@ -48,7 +48,8 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a) {
 // ---------------------------------------------------------------------------
 // V = 0 is the plan of every kernel.  V = 1 trades registers for fewer
 // Stockham passes (fewer LDS exchanges and workgroup barriers); it differs
-// only where measured to pay: 256 = 16 * 16 for the FP32 column kernels.
+// only where measured to pay: 256 = 16 * 16 and 128 = 16 * 8 for the FP32
+// column kernels and x kernels.
 // V = 2 goes the other way (more threads per transform, fewer registers).
 template <int N, int V = 0> struct Plan;
 template <> struct Plan<16, 0>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
@@ -63,6 +64,7 @@ template <> struct Plan<384, 0> { static constexpr int E = 12, NP = 5; static co
 template <> struct Plan<512, 0> { static constexpr int E = 8,  NP = 3; static constexpr int R[3] = {8, 8, 8}; };
 template <int N> struct Plan<N, 1> : Plan<N, 0> {};
 template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 16}; };
+template <> struct Plan<128, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 8}; };
 
 template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 

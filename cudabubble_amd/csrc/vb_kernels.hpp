@@ -34,9 +34,10 @@ template <typename T, int N> constexpr int xlanes() { return XWG / TFof<N, xplan
 // gains more in latency hiding than it loses); FP64 keeps the full budget.
 template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
 
-// Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256:
-// 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>).
-template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && N == 256) ? 1 : 0; }
+// Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256 and
+// 128: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>), 16 x 8 instead of
+// 8 x 4 x 4 (with the x kernel's, 12 x 128^3 batched 3.74 -> 3.67 ms per step).
+template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && (N == 256 || N == 128)) ? 1 : 0; }
 
 // Column tile width of the y passes and the z Laplacian (columns that are
 // contiguous in memory): enough columns for >= 128-byte segments and >= 256
@@ -187,6 +188,7 @@ struct XaArgs {
   cpx<T>* W;         // out (Euler only): x-forward of f, times inv_n
   cpx<T>* Ts;        // out (k_xrow): x-forward of s, times inv_n
   double* part;      // [gridDim.x][3]
+  double* dcpart;    // k_xrow, FP32: [gridDim.x][2] FP64 sums of tmp1 and eps^3 s, or nullptr
   const cpx<T>* tw;
   long long npairs;
   double eps, kappa, D, inv_n;
@@ -270,11 +272,16 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
 // Plan of k_xrow: at 256 the 16 x 16 plan (one LDS exchange per transform
 // instead of two) ran 71 -> 65 us in FP32 (then 204 registers, two waves per
 // SIMD, against 116 and four for 8 x 8 x 4) and raised the FP64 rate 3 % (250
-// registers, two waves).
-template <typename T, int N> constexpr int xrplan() { return N == 256 ? 1 : xplan<T, N>(); }
-// the other plan a kernel schedule can select (the same where no 16 x 16
+// registers, two waves); at 128 in FP32 the 16 x 8 plan (one exchange instead
+// of two).  The FP64 x kernel at 128 keeps 8 x 4 x 4.
+template <typename T, int N> constexpr int xrplan() {
+  return (N == 256 || (N == 128 && sizeof(T) == 4)) ? 1 : xplan<T, N>();
+}
+// the other plan a kernel schedule can select (the same where no two-pass
 // plan exists)
-template <typename T, int N> constexpr int xrplan_alt() { return xrplan<T, N>() == 1 ? 0 : (N == 256 ? 1 : xrplan<T, N>()); }
+template <typename T, int N> constexpr int xrplan_alt() {
+  return xrplan<T, N>() == 1 ? 0 : ((N == 256 || (N == 128 && sizeof(T) == 4)) ? 1 : xrplan<T, N>());
+}
 // threads of a k_xrow workgroup: as many rows as the other x kernels'
 // workgroups (two per row pair), so the partial sums line up
 template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() { return 2 * xlanes<T, N>() * TFof<N, XP>(); }
@@ -318,6 +325,12 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(row), a.Q + a.rl.off(row), j + TF * e);
   fft<T, N, +1, XP>(x, j, a.tw, lds);   // v + i lap v
   double acc[3] = {0.0, 0.0, 0.0};
+  // FP32: the k = 0 coefficients of tmp1 and eps^3 s are sums over the whole
+  // grid of O(1) values that largely cancel, which an FP32 transform rounds
+  // at ~1e-7 of the sum of magnitudes; summed here in FP64 from the same
+  // values, they replace the transform's in the stage combine
+  constexpr bool DCS = sizeof(T) == 4 && !SUMS;
+  double dcs[2] = {0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     // in the work precision: v and lap v come out of a T transform, so an
@@ -329,11 +342,20 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
     const T t1 = v * (w - T(kappa));                                    // nld_kernel1, Calc.cu:270
     const T sv = T(D) * t1 + (T(3) * v * v - T(D + 1)) * (T(eps) * f);  // eps^3 s, Calc.cu:282
     if (valid) { acc[0] += double(v); acc[1] += double(w) * double(w); acc[2] += double(f) * double(f); }
+    if constexpr (DCS) {
+      if (valid) { dcs[0] += double(t1); dcs[1] += double(sv); }
+    }
     x[e] = mk<T>(t1, sv);
   }
   if constexpr (SUMS) {
     block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
     return;
+  }
+  if constexpr (DCS) {
+    if (a.dcpart) {
+      __shared__ double red2[2 * (NT / 64)];
+      block_sum_store<2, NT>(dcs, red2, a.dcpart + 2 * (long long)blockIdx.x);
+    }
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
   unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
@@ -536,7 +558,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
     // their workgroups are dispatched first and overlap the column work
     if (blockIdx.y == 0) {
       double* red = reinterpret_cast<double*>(sm);   // the column LDS, unused here
-      finalize_body<TB * TF, 2>(a.fin, red, red + 4 * ((TB * TF) / 64), blockIdx.x, a.fin_on);
+      finalize_body<TB * TF, 2>(a.fin, red, red + 6 * ((TB * TF) / 64), blockIdx.x, a.fin_on);
     }
     return;
   }
@@ -685,8 +707,8 @@ __global__ void __launch_bounds__((TB * TFof<N, V>()), 4) k_zlap(ZlArgs<T> a0, c
   if ((int)blockIdx.x < a.fin_on) {
     // the integrals only read partials of kernels that ran before this one;
     // their workgroups are dispatched first and overlap the column work
-    __shared__ double red[4 * ((TB * TF) / 64)];
-    __shared__ double tot[5];
+    __shared__ double red[6 * ((TB * TF) / 64)];
+    __shared__ double tot[7];
     finalize_body<TB * TF>(a.fin, red, tot, blockIdx.x, a.fin_on);
     return;
   }
@@ -888,7 +910,15 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
     const int kz = j + TF * e;
     const double k2 = double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
     const double at = -(l2c + a.eta * k2) * ie;   // on F(tmp1)
-    double pr = at * double(x[e].x) - a.eta * double(x2[e].x);
+    double tr = double(x[e].x), sr = double(x2[e].x);
+    if constexpr (sizeof(T) == 4) {
+      // the k = 0 coefficients summed in FP64 by the x kernel (k_xrow)
+      if (e == 0 && id.q == 0 && j == 0) {
+        tr = a.sc->dc_t1;
+        sr = a.sc->dc_s * (ie * ie * ie);
+      }
+    }
+    double pr = at * tr - a.eta * sr;
     const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
     if (id.q == 0 && kz == 0) pr -= l1h;
     np[e] = mk<double>(pr, pi);
@@ -907,9 +937,11 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   }
   const int tcol = int(threadIdx.x % TB);
   constexpr bool USES_U = !(SCH == 4 && STG == 3);
-  // The last FP64 stage loads and combines its operands in two halves: all at
-  // once they need 256 registers and up to 38 more in AGPRs, one wave per SIMD
-  constexpr int NH = (sizeof(T) == 8 && STG == SCH) ? 2 : 1, EH = E / NH;
+  // The last stage loads and combines its operands in two halves in FP64, and
+  // in FP32 with 12 elements per thread (192, 384): all at once they need 256
+  // registers and more in AGPRs, one wave per SIMD; in halves two (4 x 192^3
+  // batched 4.15 -> 3.98 ms per step)
+  constexpr int NH = ((sizeof(T) == 8 || E >= 12) && STG == SCH) ? 2 : 1, EH = E / NH;
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll

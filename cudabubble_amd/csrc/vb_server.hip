@@ -44,6 +44,20 @@ struct Running {
   std::deque<double> hist;   // energies of the last window+1 accepted states
 };
 
+// A finished request's solver, kept with its memory for the next request of
+// the same grid, precision and buffer set (vb::SolverBase::reset).
+struct Idle {
+  std::unique_ptr<vb::SolverBase> s;
+  int stream;
+  size_t bytes;
+  vb_params p;
+};
+
+bool same_slot(const vb_params& a, const vb_params& b) {
+  const bool wa = a.scheme == VB_EULER || a.shape_adjust, wb = b.scheme == VB_EULER || b.shape_adjust;
+  return a.nx == b.nx && a.nz == b.nz && a.precision == b.precision && (wa || !wb);
+}
+
 double now_s(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -61,7 +75,8 @@ struct vb_server {
   std::vector<vb_result> done;
   std::map<long long, std::vector<double>> phis;
   std::vector<Running> running;   // worker thread only
-  size_t used = 0;
+  std::deque<Idle> idle;          // worker thread only: the slot pool, oldest first
+  size_t used = 0;                // bytes of every solver held, running or idle
   int nrun = 0;
   bool stop = false;
   int worker_error = VB_OK;
@@ -75,6 +90,7 @@ struct vb_server {
     cv.notify_all();
     if (worker.joinable()) worker.join();
     running.clear();
+    idle.clear();
     for (auto s : streams) {
       (void)hipStreamSynchronize(s);
       (void)hipStreamDestroy(s);
@@ -88,7 +104,9 @@ struct vb_server {
     res.steps = r.rep.steps;
     res.evaluations = r.rep.evaluations;
     res.rejections = r.rep.rejections;
-    r.s->query(&res.E, &res.V, &res.A, &res.Eb);
+    // the last accepted attempt left the final state's integrals in host
+    // memory; a query (a device round trip) only after a driver exit
+    if (!r.s->accepted_values(&res.E, &res.V, &res.A, &res.Eb)) r.s->query(&res.E, &res.V, &res.A, &res.Eb);
     r.s->state(nullptr, nullptr, &res.time_step, &res.time_current);
     res.t_submit = r.req.t_submit;
     res.t_admit = r.t_admit;
@@ -99,9 +117,8 @@ struct vb_server {
     }
     res.t_done = now_s(t0);
     stream_load[(size_t)r.stream] -= (long long)r.req.p.nx * r.req.p.nx * r.req.p.nz;
-    r.s.reset();
+    idle.push_back(Idle{std::move(r.s), r.stream, r.bytes, r.req.p});   // memory kept for reuse
     std::lock_guard<std::mutex> g(mu);
-    used -= r.bytes;
     nrun--;
     if (cfg.keep_phi) phis[res.id] = std::move(phi);
     done.push_back(res);
@@ -121,30 +138,51 @@ struct vb_server {
     return false;
   }
 
+  // Admission in arrival order.  A request takes an idle solver of its slot
+  // kind if there is one (no allocation); otherwise a new one, after idle
+  // solvers are released, oldest first, to make room in the memory budget.
   void admit() {
     for (;;) {
       Request q;
-      size_t bytes;
+      size_t bytes = 0;
+      int reuse = -1;
       {
         std::lock_guard<std::mutex> g(mu);
         if (pending.empty() || (int)running.size() >= cfg.max_active) return;
-        bytes = vb::solver_footprint(pending.front().p);
-        if (used + bytes > cfg.memory_budget && !running.empty()) return;   // wait for memory
+        const vb_params& fp = pending.front().p;
+        for (size_t i = 0; i < idle.size() && reuse < 0; ++i)
+          if (same_slot(idle[i].p, fp)) reuse = (int)i;
+        if (reuse < 0) {
+          bytes = vb::solver_footprint(fp);
+          while (used + bytes > cfg.memory_budget && !idle.empty()) {
+            used -= idle.front().bytes;
+            idle.pop_front();     // releases its device memory
+          }
+          if (used + bytes > cfg.memory_budget && !running.empty()) return;   // wait for memory
+          used += bytes;
+        }
         q = std::move(pending.front());
         pending.pop_front();
-        used += bytes;
         nrun++;
       }
-      const int k = int(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
       Running r;
-      r.stream = k;
-      r.bytes = bytes;
       r.t_admit = now_s(t0);
       r.rep.energy = NAN;
       int fail = VB_OK;
       try {
-        r.s = vb::make_solver(q.p, streams[(size_t)k]);
-        r.s->set_phi(q.phi0.data());
+        if (reuse >= 0) {
+          Idle it = std::move(idle[(size_t)reuse]);
+          idle.erase(idle.begin() + reuse);
+          r.stream = it.stream;
+          r.bytes = it.bytes;
+          r.s = std::move(it.s);
+          if (!r.s->reset(q.p, q.phi0.data())) throw std::bad_alloc();   // same_slot guarantees a fit
+        } else {
+          r.stream = int(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
+          r.bytes = bytes;
+          r.s = vb::make_solver(q.p, streams[(size_t)r.stream]);
+          r.s->set_phi(q.phi0.data());
+        }
       } catch (const vb::HipError& e) {
         fail = e.err == hipErrorOutOfMemory ? VB_ERR_OOM : VB_ERR_HIP;
       } catch (const std::bad_alloc&) {
@@ -159,12 +197,12 @@ struct vb_server {
         res.t_admit = r.t_admit;
         res.t_done = now_s(t0);
         std::lock_guard<std::mutex> g(mu);
-        used -= bytes;
+        used -= reuse >= 0 ? r.bytes : bytes;
         nrun--;
         done.push_back(res);
         continue;
       }
-      stream_load[(size_t)k] += (long long)q.p.nx * q.p.nx * q.p.nz;
+      stream_load[(size_t)r.stream] += (long long)q.p.nx * q.p.nx * q.p.nz;
       q.phi0.clear();
       q.phi0.shrink_to_fit();
       r.req = std::move(q);

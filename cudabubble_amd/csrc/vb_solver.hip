@@ -33,8 +33,8 @@ constexpr int kFinThreads = 1024;   // one block; wide so the partial sums load 
 template <bool TAB>
 __global__ void __launch_bounds__(kFinThreads) k_finalize(FinArgs a0, const FinArgs* tab) {
   const FinArgs a = TAB ? tab[blockIdx.z] : a0;
-  __shared__ double red[4 * (kFinThreads / 64)];
-  __shared__ double tot[5];
+  __shared__ double red[6 * (kFinThreads / 64)];
+  __shared__ double tot[7];
   finalize_body<kFinThreads>(a, red, tot, 0, 1);
 }
 
@@ -106,6 +106,7 @@ FinArgs fin_args(const double* xpart, int nxb, const double* zpart, const double
   a.xpart = xpart;
   a.zpart = zpart;
   a.dc = dc;
+  a.dcpart = nullptr;
   a.sc = sc;
   a.fs = fs;
   a.nxb = nxb;
@@ -161,6 +162,7 @@ struct Solver final : SolverBase {
   DBuf<cpx<T>> pre, Na, Nb, Vh;          // stage spectra kept for later stages (group-major)
   DBuf<T> v, f;
   DBuf<double> xpart, zpart, phid, dc;
+  DBuf<double> xdc;                      // FP32-mixed: the x kernel's FP64 sums of tmp1 and eps^3 s
   DBuf<Scal> scal;
   DBuf<Scal> scal_keep;                  // the state's integrals across the drift double step
   DBuf<FinScratch> fin_scratch;
@@ -178,11 +180,25 @@ struct Solver final : SolverBase {
   int nxb = 0, nzb = 0;
   long long npairs = 0;
   int mmax = 0;
+  // ETD coefficient tables: a fixed pool of slots allocated with the solver.
+  // A (scheme, h) table is built on the host at its first use, into pinned
+  // staging, and copied on the solver's stream; when the pool is full the
+  // least recently used slot is refilled and the captured graphs that read it
+  // are dropped.  Stepping never allocates, frees or waits for the device.
+  static constexpr int kLutSlots = 24;
   struct Lut {
-    DBuf<double2> c1;
-    DBuf<double4> c2;
+    double2* c1;
+    double4* c2;
+    int sch;        // 0: empty
+    double h;
+    unsigned long long used;
   };
-  std::map<std::pair<int, double>, std::unique_ptr<Lut>> luts;
+  Lut slots[kLutSlots] = {};
+  unsigned long long lut_clock = 0;
+  DBuf<double2> lut_c1;
+  DBuf<double4> lut_c2;
+  double* lut_stage = nullptr;      // pinned, (mmax + 1) * 6 doubles
+  hipEvent_t stage_ev = nullptr;    // the last copy out of lut_stage
 
   // pipeline state
   int cur = 0;
@@ -231,8 +247,9 @@ struct Solver final : SolverBase {
   PipeState fixed_post{};        // state after the last fixed iteration
   bool have_fixed_post = false;
 
-  Solver(const vb_params& pp, hipStream_t shared) {
-    p = pp;
+  // geometry and physical constants of p (the buffer layout depends on nx,
+  // nz and the kernel schedules only)
+  void setup_geo() {
     g = make_geo(p.nx, p.nz, p.len * (2.0 * 3.1415926535897932384626433832795));
     ph.eps = p.eps;
     ph.eta = p.eta;
@@ -241,10 +258,6 @@ struct Solver final : SolverBase {
     ph.D = p.D;
     ph.M1 = p.M1;
     ph.M2 = p.M2;
-    ox = get_ops<T>(p.nx);
-    oz = get_ops<T>(p.nz);
-    oxd = get_ops<double>(p.nx);
-    ozd = get_ops<double>(p.nz);
     // padded, 128-byte aligned spectrum rows when every column tile width in
     // use divides nx/2 (see Geo)
     bool tiles = true;
@@ -258,6 +271,15 @@ struct Solver final : SolverBase {
       g.pp = (long long)g.ny * g.pitch + 16;
       g.ncp = (long long)g.nz * g.pp;
     }
+  }
+
+  Solver(const vb_params& pp, hipStream_t shared) {
+    p = pp;
+    ox = get_ops<T>(p.nx);
+    oz = get_ops<T>(p.nz);
+    oxd = get_ops<double>(p.nx);
+    ozd = get_ops<double>(p.nz);
+    setup_geo();
     if (shared) {
       st = shared;
     } else {
@@ -284,6 +306,7 @@ struct Solver final : SolverBase {
     nxb = int((npairs + ox->xlanes - 1) / ox->xlanes);
     nzb = int(zgroups(g, oz->ytile));
     xpart.alloc((size_t)nxb * 3);
+    if (kMixed) xdc.alloc((size_t)nxb * 2);
     zpart.alloc((size_t)nzb);
     scal.alloc(1);
     scal_keep.alloc(1);
@@ -297,6 +320,14 @@ struct Solver final : SolverBase {
     upload_twiddles(twxd, p.nx);
     upload_twiddles(twzd, p.nz);
     mmax = mmax_of(p.nx, p.ny, p.nz);
+    lut_c1.alloc((size_t)kLutSlots * (mmax + 1));
+    lut_c2.alloc((size_t)kLutSlots * (mmax + 1));
+    for (int k = 0; k < kLutSlots; ++k) {
+      slots[k].c1 = lut_c1.p + (size_t)k * (mmax + 1);
+      slots[k].c2 = lut_c2.p + (size_t)k * (mmax + 1);
+    }
+    VB_HIP(hipHostMalloc((void**)&lut_stage, sizeof(double) * 6 * (size_t)(mmax + 1), hipHostMallocDefault));
+    VB_HIP(hipEventCreateWithFlags(&stage_ev, hipEventDisableTiming));
     time_step = p.time_step;
     alpha = p.alpha;
     beta = p.beta;
@@ -311,6 +342,8 @@ struct Solver final : SolverBase {
     for (auto& G : graphs)
       if (G.x) (void)hipGraphExecDestroy(G.x);
     if (att_ev) (void)hipEventDestroy(att_ev);
+    if (stage_ev) (void)hipEventDestroy(stage_ev);
+    if (lut_stage) (void)hipHostFree(lut_stage);
     flush_prof();
     for (auto e : evpool) (void)hipEventDestroy(e);
     if (hs) (void)hipHostFree(hs);
@@ -331,24 +364,30 @@ struct Solver final : SolverBase {
   }
 
   const Lut* lut(int sch, double h) {
-    auto key = std::make_pair(sch, h);
-    auto it = luts.find(key);
-    if (it != luts.end()) return it->second.get();
-    if (luts.size() > 64) {
-      // captured graphs hold table addresses: drop them with the tables
-      VB_HIP(hipStreamSynchronize(st));
+    Lut* hit = nullptr;
+    for (auto& sl : slots)
+      if (sl.sch == sch && sl.h == h) hit = &sl;
+    if (hit) {
+      hit->used = ++lut_clock;
+      return hit;
+    }
+    Lut* sl = &slots[0];
+    for (auto& c : slots)
+      if (c.sch == 0 || (sl->sch != 0 && c.used < sl->used)) sl = &c;
+    if (sl->sch != 0) {
+      // captured graphs read this slot's table: drop those of its step
       for (auto& G : graphs)
-        if (G.x) {
+        if (G.x && G.pre.dt == sl->h) {
           VB_HIP(hipGraphExecDestroy(G.x));
           G.x = nullptr;
         }
-      luts.clear();
     }
     std::vector<Coef> tab;
     build_lut(tab, mmax, sch, p.phi_lut_mode, h, g.kfac2, p.eps, p.kappa, p.D);
     // compact per-stage arrays: {e1, q} and {e2, A s, 2 B s, C s} (ETDRK2: {e, q1}, {c2})
-    std::vector<double2> h1(tab.size());
-    std::vector<double4> h2(tab.size());
+    VB_HIP(hipEventSynchronize(stage_ev));   // the staging buffer's previous copy is done
+    double2* h1 = reinterpret_cast<double2*>(lut_stage);
+    double4* h2 = reinterpret_cast<double4*>(lut_stage + 2 * (size_t)(mmax + 1));
     for (size_t m = 0; m < tab.size(); ++m) {
       h1[m].x = tab[m].c[0];
       h1[m].y = tab[m].c[1];
@@ -362,14 +401,13 @@ struct Solver final : SolverBase {
         h2[m].w = tab[m].c[5];
       }
     }
-    auto b = std::make_unique<Lut>();
-    b->c1.alloc(tab.size());
-    b->c2.alloc(tab.size());
-    VB_HIP(hipMemcpy(b->c1.p, h1.data(), sizeof(double2) * h1.size(), hipMemcpyHostToDevice));
-    VB_HIP(hipMemcpy(b->c2.p, h2.data(), sizeof(double4) * h2.size(), hipMemcpyHostToDevice));
-    const Lut* r = b.get();
-    luts.emplace(key, std::move(b));
-    return r;
+    VB_HIP(hipMemcpyAsync(sl->c1, h1, sizeof(double2) * tab.size(), hipMemcpyHostToDevice, st));
+    VB_HIP(hipMemcpyAsync(sl->c2, h2, sizeof(double4) * tab.size(), hipMemcpyHostToDevice, st));
+    VB_HIP(hipEventRecord(stage_ev, st));
+    sl->sch = sch;
+    sl->h = h;
+    sl->used = ++lut_clock;
+    return sl;
   }
 
   // ---- profiling ---------------------------------------------------------
@@ -467,11 +505,13 @@ struct Solver final : SolverBase {
     cpx<T>* tt = set ? Tt1.p : Tt.p;
     cpx<T>* ts = set ? Ts1.p : Ts.p;
     L("y_inv2", [&] { ox->ypass(P.p, Q.p, 2, +1, g, twx.p, nullptr, st); });
-    XaArgs<T> a;
+    XaArgs<T> a{};
     a.P = P.p;
     a.Q = Q.p;
     a.v = v.p;
     a.f = f.p;
+    // FP32-mixed ETD prefix: FP64 sums of tmp1 and eps^3 s for the k = 0 modes
+    a.dcpart = (kMixed && !withf) ? xdc.p : nullptr;
     // ETD: tmp1 and s go to P, Q in place (each lane reads its two rows of P
     // and Q before it writes them); Euler: tmp1 to a scratch, f to W
     a.Tt = withf ? tt : P.p;
@@ -488,7 +528,8 @@ struct Solver final : SolverBase {
     L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf ? XA_WITHF : XA_WITHS, st); });
     // the integrals of this field (xa's and the z kernel's partials) are
     // evaluated by extra workgroups of the next column pass
-    const FinArgs fin = fin_args(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, fin_scratch.p, g, ph, mode);
+    FinArgs fin = fin_args(xpart.p, nxb, zpart.p, dc.p, nzb, scal.p, fin_scratch.p, g, ph, mode);
+    fin.dcpart = a.dcpart;
     if (withf) {
       spectral_lap(W.p, &fin);
     } else {
@@ -552,8 +593,8 @@ struct Solver final : SolverBase {
     z.Q = Q.p;
     z.part = zpart.p;
     z.dc = dc.p;
-    z.c1 = lt->c1.p;
-    z.c2 = lt->c2.p;
+    z.c1 = lt->c1;
+    z.c2 = lt->c2;
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
@@ -810,6 +851,7 @@ struct Solver final : SolverBase {
   bool att_pending = false;
   bool iter_open = false;      // dt chosen for the current iteration
   bool cur_E_known = false;    // cur_E is the energy of uh[cur] with the current targets
+  bool hs_state = false;       // hs holds the integrals of uh[cur] (an accepted attempt's)
   double cur_E = 0.0, att_eng0 = 0.0;
   int att_keep = 0;
 
@@ -818,6 +860,7 @@ struct Solver final : SolverBase {
   static constexpr bool kMixed = sizeof(T) == 4;
 
   void adaptive_issue(vb_step_report& r) override {
+    hs_state = false;
     if (!started) start();
     if (kMixed) need64();   // allocations stay outside graph capture
     if (!iter_open) {
@@ -888,6 +931,7 @@ struct Solver final : SolverBase {
     att_pending = false;
     r.evaluations++;
     if (hs->E > att_eng0) {
+      hs_state = false;
       cur = att_keep;   // the state is untouched; only its prefix is redone
       prefixed = prefixed_f = prepped = false;
       time_step /= 2;
@@ -897,6 +941,7 @@ struct Solver final : SolverBase {
     iter_open = false;
     cur_E = hs->E;
     r.energy = hs->E;
+    hs_state = true;
     if (att_eng0 < 1e-3) r.status = VB_FINISHED;          // belastic.cpp:727-732
     else if (hs->nonfinite) r.status = VB_ERR_NONFINITE;  // belastic.cpp:733-740
     if (r.status != VB_OK) return 1;
@@ -909,6 +954,7 @@ struct Solver final : SolverBase {
   }
 
   void launch_steps(int n, vb_step_report* rep) override {
+    hs_state = false;
     if (!started) start();
     cur_E_known = false;
     if (time_step < 1e-23) {
@@ -930,10 +976,12 @@ struct Solver final : SolverBase {
   }
 
   void read_scal() {
+    hs_state = false;
     VB_HIP(hipMemcpyAsync(hs, scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
   }
   void read_scal64() {
+    hs_state = false;
     VB_HIP(hipMemcpyAsync(hs, scal64.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
   }
@@ -1187,6 +1235,7 @@ struct Solver final : SolverBase {
 
   void set_phi(const double* host) override {
     cur_E_known = false;
+    hs_state = false;
     iter_open = false;
     VB_HIP(hipMemcpyAsync(phid.p, host, sizeof(double) * (size_t)g.nr, hipMemcpyHostToDevice, st));
     L("x_r2c", [&] { oxd->xr2c_d(phid.p, S.p, npairs, twxd.p, 1.0 / double(g.nr), g, st); });
@@ -1214,6 +1263,7 @@ struct Solver final : SolverBase {
 
   void set_spec(const double* host) override {
     cur_E_known = false;
+    hs_state = false;
     iter_open = false;
     const size_t nh = (size_t)g.nx / 2 + 1;
     std::vector<cpx<double>> h((size_t)g.ncp, cpx<double>{0.0, 0.0});
@@ -1261,8 +1311,51 @@ struct Solver final : SolverBase {
     return sum / double(g.nr) * g.L3;   // Calc.cu:250,260 with a = 0, m = 1
   }
 
+  bool accepted_values(double* E, double* V, double* A, double* Eb) const override {
+    if (!hs_state) return false;
+    if (E) *E = hs->E;
+    if (V) *V = hs->V;
+    if (A) *A = hs->A;
+    if (Eb) *Eb = hs->Eb;
+    return true;
+  }
+
+  bool reset(const vb_params& pp, const double* phi0) override {
+    if (pp.nx != p.nx || pp.nz != p.nz || pp.precision != p.precision) return false;
+    if ((pp.scheme == VB_EULER || pp.shape_adjust) && !W.p) return false;
+    // the last work of the previous solve has completed (its attempt's event
+    // or a synchronising call); graphs and tables hold the old parameters
+    for (auto& G : graphs)
+      if (G.x) {
+        VB_HIP(hipGraphExecDestroy(G.x));
+        G.x = nullptr;
+      }
+    graph_next = 0;
+    for (auto& sl : slots) sl.sch = 0;
+    flush_prof();
+    prof = false;
+    p = pp;
+    setup_geo();
+    cur = 0;
+    prepped = prefixed = prefixed_f = false;
+    started = false;
+    time_step = p.time_step;
+    time_current = 0.0;
+    steps = 0;
+    alpha = alpha0 = p.alpha;
+    beta = beta0 = p.beta;
+    have_fixed_post = false;
+    rec_evals = 0;
+    att_pending = iter_open = cur_E_known = hs_state = false;
+    info_extra[0] = info_extra[1] = info_extra[2] = 0.0;
+    set_targets_device();
+    set_phi(phi0);
+    return true;
+  }
+
   void evaluate(double h) override {
     cur_E_known = false;
+    hs_state = false;
     const int out = eval(h);
     cur = out;
   }
@@ -1283,7 +1376,7 @@ size_t solver_footprint(const vb_params& p) {
   size_t bytes = 3 * nc * 16 + work * nc * cs + (euler ? 2 * nr * rs : 0) + nr * 8;
   if (mixed) bytes += 2 * nc * 16;   // FP64 integrals of the state (P64, Q64)
   const size_t mm = (size_t)mmax_of(p.nx, p.nx, p.nz) + 1;
-  return bytes + 2 * mm * 48 + ((size_t)4 << 20);
+  return bytes + (size_t)Solver<double>::kLutSlots * mm * 48 + ((size_t)4 << 20);
 }
 
 }  // namespace vb

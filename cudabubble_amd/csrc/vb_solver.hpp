@@ -68,6 +68,15 @@ struct SolverBase {
   virtual void adaptive_issue(vb_step_report& rep) = 0;
   virtual bool attempt_ready() = 0;
   virtual int adaptive_complete(vb_step_report& rep) = 0;
+  // The request loop's slot pool: reuse this solver's memory for another solve
+  // of the same grid, precision and buffer set (false if p needs others).
+  // Captured graphs and tables of the old parameters are dropped; phi0 is
+  // uploaded on the solver's stream.
+  virtual bool reset(const vb_params& p, const double* phi0) = 0;
+  // E, V, A and E_b of the current state as the last accepted adaptive attempt
+  // left them in host memory (FP64 in both precisions), without a device
+  // round trip; false if no attempt was accepted or the state changed since.
+  virtual bool accepted_values(double* E, double* V, double* A, double* Eb) const = 0;
   // extra info stored in the checkpoint (.info): alpha3, alpha_1, beta_1
   double info_extra[3] = {0, 0, 0};
 };

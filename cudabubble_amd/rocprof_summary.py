@@ -44,8 +44,8 @@ def solver_name(mangled):
     a = args[1:] + [None] * 6
     if kind == "xa":   # <T, N, WITHF, TAB>
         return ("xa_f" if a[1] == "true" else "xa_plain"), prec
-    if kind == "xrow":   # the x kernel of the ETD stages
-        return "xa", prec
+    if kind == "xrow":   # the x kernel of the ETD stages: <T, N, TAB, XP, SUMS>
+        return ("xrow_sums" if a[3] == "true" else "xa"), prec
     if kind == "xsums":
         return "xa_sums", prec
     if kind == "ypass":

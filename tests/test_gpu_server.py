@@ -86,3 +86,25 @@ def test_energy_drop_exit(vb, gpu):
     p, phi = request(vb, 32, 1, 50, 1)
     res, _ = run(vb, [(p, phi)], converge_window=5, converge_tol=1.0)
     assert res[0]["status"] == 0 and res[0]["steps"] == 6
+
+
+def test_slot_pool_reuse_matches_fresh_solves(vb, gpu):
+    # one slot: every request after the first reuses the previous request's
+    # solver (same grid and precision) or, when its kind differs and the budget
+    # holds one solver only, replaces it; each result is what a fresh vb_step
+    # computes for that request
+    reqs = [request(vb, 32, 1, 61, 1), request(vb, 32, 3, 62, 1, eta=1.5),
+            request(vb, 64, 4, 63, 1), request(vb, 32, 2, 64, 1)]
+    reqs[1][0].eps *= 1.5           # other eps: other tables and constants in the reused solver
+    reqs[3][0].M1 = 3e3
+    budget = vb.footprint(reqs[2][0]) + vb.footprint(reqs[0][0]) // 2   # a 64^3 solver or a 32^3 one
+    res, phis = run(vb, reqs, max_active=1, streams=1, max_steps=12, memory_budget=budget)
+    for i, (p, phi) in enumerate(reqs):
+        flush_print("slot pool: fresh solve", i)
+        s = vb.Solver(p, phi)
+        r = s.step(12)
+        assert (r["steps"], r["evaluations"], r["rejections"]) == \
+            (res[i]["steps"], res[i]["evaluations"], res[i]["rejections"]), i
+        assert np.array_equal(s.get_phi(), phis[i]), i
+        assert s.query() == (res[i]["E"], res[i]["V"], res[i]["A"], res[i]["Eb"]), i
+        s.close()

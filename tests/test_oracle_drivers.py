@@ -118,3 +118,62 @@ def test_reference_etdrk4_small_z_cancellation(oracle_mod):
     # at h = 1e-6 (|z| >= 6e-6) the same step is within 15% of the Euler step scaled by 100
     step = np.abs(o.etd4rk(d0, 1e-6) - d0).max()
     assert abs(step / (100 * np.abs(de - d0).max()) - 1) < 0.15
+
+
+def _fd_gradient(o, d, psi, h=1e-5):
+    """directional derivative of the oracle's constrained energy along psi"""
+    Ep = o.energy(d + h * psi)[0]
+    Em = o.energy(d - h * psi)[0]
+    return (Ep - Em) / (2 * h)
+
+
+@pytest.mark.parametrize("M1,M2", [(1e3, 0.0), (0.0, 2e3), (1e3, 2e3)])
+def test_euler_rhs_against_the_energy_gradient(oracle_mod, M1, M2):
+    """vbo_nld1 through the Euler right-hand side r = lambda1/2 + lambda2 f + eta g0
+    (belastic.cpp:52-64, Calc.cu:410-426), pinned per coefficient by the
+    functional derivative of E = eta E_b + M1/2 (V - alpha)^2 + M2/2 (A - beta)^2
+    taken by finite differences of the oracle's own energy: dE_b/dphi = g0 exactly
+    (the spectral Laplacian is symmetric), dV/dphi = 1/2 and dA/dphi = 3/(2 sqrt2) f
+    (along a smooth direction, whose Nyquist modes vanish).  So
+    <r, psi> = dE(psi) - lambda2 (3/(2 sqrt2) - 1) <f, psi>: the reference's Euler
+    step weights the area term by lambda2, not by its gradient's 3/(2 sqrt2) lambda2.
+    eta = 1.7, so a dropped eta, a lost 1/2 on lambda1 or a 3/(2 sqrt2) on lambda2
+    each break the balance."""
+    n = 16
+    eta = 1.7
+    d = ellipsoid(n, EPS)
+    o = oracle_mod.Oracle(n, eps=EPS, eta=eta, M1=M1, M2=M2)
+    V0, A0 = o.vol(d), o.surface(d)
+    o.set_targets(0.97 * V0, 1.04 * A0)
+    X, Y, Z = grid(n)
+    psi = np.cos(X) * np.sin(2 * Y) + 0.5 * np.cos(Z - 0.3) + 0.25
+    L3 = (2 * np.pi) ** 3
+    w = L3 / d.size                        # the discrete integral: sum * L^3 / N
+    r = o.sum_fgq(d)
+    f = o.f(d)
+    lam2 = M2 * (A0 - 1.04 * A0)
+    dE = _fd_gradient(o, d, psi)
+    lhs = w * np.sum(r * psi)
+    rhs = dE - lam2 * (3 / (2 * np.sqrt(2)) - 1) * w * np.sum(f * psi)
+    assert rel(lhs, rhs) < 1e-6, (lhs, rhs)
+    # the terms one at a time, for the record of their sizes
+    g0 = o.g0(d)
+    parts = [w * np.sum(eta * g0 * psi), M1 * (V0 - 0.97 * V0) / 2 * w * np.sum(psi), lam2 * w * np.sum(f * psi)]
+    assert rel(lhs, sum(parts)) < 1e-12
+    # every active term is at least 1e-4 of the total, so a dropped eta (41% of its
+    # term), 1/2 (50%) or an extra 3/(2 sqrt2) (6%) moves the balance by at least
+    # 6e-6, six times the tolerance (measured: 5e-5 to 1, against at most 2e-8)
+    for k, q in enumerate(parts):
+        if abs(q) > 0:
+            assert abs(q) > 1e-4 * abs(lhs), (k, parts)
+
+
+def test_nld1_coefficients_one_at_a_time(oracle_mod):
+    """vbo_nld1 on constant fields: f = g = 0 leaves lambda1/2; f = 1, g = 0 adds
+    lambda2 (no 3/(2 sqrt2)); f = 0, g = 1 adds eta (Calc.cu:417, eta = 2.5)."""
+    n = 8
+    o = oracle_mod.Oracle(n, eps=EPS, eta=2.5)
+    z, one = np.zeros((n, n, n)), np.ones((n, n, n))
+    assert np.all(o.nld1(z, z, 3.0, 7.0) == 1.5)
+    assert np.all(o.nld1(one, z, 3.0, 7.0) == 1.5 + 7.0)
+    assert np.all(o.nld1(z, one, 3.0, 7.0) == 1.5 + 2.5)

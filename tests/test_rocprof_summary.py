@@ -10,7 +10,8 @@ from cudabubble_amd import rocprof_summary as rs
 
 def test_solver_names():
     cases = {
-        "void vb::k_xrow<float, 256, false, 1>(vb::XaArgs<float>, vb::XaArgs<float> const*)": ("xa", "fp32"),
+        "void vb::k_xrow<float, 256, false, 1, false>(vb::XaArgs<float>, vb::XaArgs<float> const*)": ("xa", "fp32"),
+        "void vb::k_xrow<double, 256, false, 0, true>(vb::XaArgs<double>, vb::XaArgs<double> const*)": ("xrow_sums", "fp64"),
         "void vb::k_xa<double, 256, true, false>(vb::XaArgs<double>, vb::XaArgs<double> const*)": ("xa_f", "fp64"),
         "void vb::k_xa<double, 256, false, false>(vb::XaArgs<double>, vb::XaArgs<double> const*)": ("xa_plain", "fp64"),
         "void vb::k_xsums<float, 256, false>(vb::XaArgs<float>, vb::XaArgs<float> const*)": ("xa_sums", "fp32"),
