@@ -172,6 +172,7 @@ def lib():
     L.vb_schedule_describe.argtypes = [I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     L.vb_autotune.argtypes = [I, I, I, I, C.POINTER(TuneReport)]
     L.vb_copy_bandwidth.argtypes = [I, C.c_size_t, I, C.POINTER(D), C.POINTER(D), I]
+    L.vb_intf_static.argtypes = [P, C.c_longlong, D, D, D, P, P]
     _LIB = L
     return L
 
@@ -561,3 +562,13 @@ def copy_bandwidth(nbytes=1 << 30, repeats=20, device=0):
                                    len(COPY_VARIANTS)), "vb_copy_bandwidth")
     return best.value, dict(zip(COPY_VARIANTS, list(each)))
 
+
+def intf_static(u, lenx, addition, multiply, val, stream=None):
+    """vb_intf_static on torch device tensors: u (float64, contiguous) and the
+    one-element float64 output val; launched on `stream` (a torch stream, or the
+    current one) without waiting and without allocating."""
+    import torch
+    assert u.dtype == torch.float64 and u.is_contiguous() and val.dtype == torch.float64
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    _check(lib().vb_intf_static(C.c_void_p(u.data_ptr()), u.numel(), float(lenx), float(addition),
+                                float(multiply), C.c_void_p(val.data_ptr()), C.c_void_p(st)), "vb_intf_static")

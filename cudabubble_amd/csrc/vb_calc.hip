@@ -6,6 +6,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <tuple>
 
 #include "vb_host.hpp"
@@ -97,19 +99,42 @@ __global__ void k_pw(PwArgs p) {
   }
 }
 
-// block partial sums for intf (Calc.cu:224-252), deterministic
-__global__ void __launch_bounds__(256) k_sum(const double* u, long long n, double* part) {
+// Block partial sums of intf (intf_kernel, Calc.cu:224-252).  Workgroup b
+// adds the elements [b kSumChunk, (b + 1) kSumChunk) in a fixed order, so the
+// result does not depend on the number of CUs (the reference's grid is twice
+// the SM count, Calc.cu:259).
+constexpr long long kSumChunk = 256 * 256;
+inline long long sum_parts(long long n) { return (n + kSumChunk - 1) / kSumChunk; }
+
+__global__ void __launch_bounds__(256) k_sum(const double* __restrict__ u, long long n, double* __restrict__ part) {
   __shared__ double red[4];
   double acc[1] = {0.0};
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) acc[0] += u[i];
+  const long long b0 = (long long)blockIdx.x * kSumChunk;
+  const long long e = b0 + kSumChunk < n ? b0 + kSumChunk : n;
+  for (long long i = b0 + threadIdx.x; i < e; i += 256) acc[0] += u[i];
   block_sum_store<1, 256>(acc, red, part + blockIdx.x);
 }
 
-__global__ void __launch_bounds__(256) k_sum_final(const double* part, int n, double* out) {
+// (sum of the nb partials / N + addition) * mL3 into out[0], mL3 = multiply L^3
+// (sumArray_kernel, Calc.cu:191-221, 255-263)
+__global__ void __launch_bounds__(256) k_intf_final(const double* __restrict__ part, int nb, double N,
+                                                    double addition, double mL3, double* __restrict__ out) {
   __shared__ double red[4];
+  __shared__ double tot[1];
   double acc[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += 256) acc[0] += part[i];
-  block_sum_store<1, 256>(acc, red, out);
+  for (int i = threadIdx.x; i < nb; i += 256) acc[0] += part[i];
+  block_sum_store<1, 256>(acc, red, tot);
+  if (threadIdx.x == 0) out[0] = (tot[0] / N + addition) * mL3;
+}
+
+// the two launches of intf: the partials of n points into part (sum_parts(n)
+// doubles), the value into val[0]
+static void launch_intf(const double* u, long long n, double L, double addition, double multiply, double* part,
+                        double* val, hipStream_t st) {
+  const long long nb = sum_parts(n);
+  hipLaunchKernelGGL(k_sum, dim3(unsigned(nb)), dim3(256), 0, st, u, n, part);
+  hipLaunchKernelGGL(k_intf_final, dim3(1), dim3(256), 0, st, part, int(nb), double(n), addition,
+                     multiply * L * L * L, val);
 }
 
 enum { ETD_1, ETD_2, ETD_41, ETD_42, ETD_43, ETD_44 };
@@ -191,7 +216,7 @@ struct CalcImpl {
     c3.alloc(nc);
     c4.alloc(nc);
     if (prec == VB_FP32_MIXED) fc1.alloc(nc);
-    part.alloc(1024);
+    part.alloc((size_t)sum_parts(g.nr));
     val.alloc(1);
     rtmp.alloc((size_t)g.nr);
     npairs = (long long)nz * nx / 2;
@@ -288,13 +313,11 @@ struct CalcImpl {
   }
   // Calc::intf (Calc.cu:255-263): (sum(u)/N + addition) * multiply * L^3
   double intf(const void* u, double addition, double multiply) {
-    const int nb = 1024;
-    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, st, (const double*)u, g.nr, part.p);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, part.p, nb, val.p);
+    launch_intf((const double*)u, g.nr, L, addition, multiply, part.p, val.p, st);
     double s = 0;
     VB_HIP(hipMemcpyAsync(&s, val.p, sizeof(double), hipMemcpyDeviceToHost, st));
     done();
-    return (s / double(g.nr) + addition) * (multiply * L * L * L);
+    return s;
   }
   const Coef* lut(int sch, double eps, double h) {
     auto key = std::make_tuple(sch, eps, h);
@@ -456,3 +479,59 @@ int vb_calc_etd44(vb_calc* c, void* d_real, const void* fd, const void* nlda, co
 int vb_calc_sync(vb_calc* c) { CALC_BODY((void)0); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The allocation-free integral (vb_intf_static).  Its block partial sums live
+// in `scratch`, a file-scope device array: every GPU the library runs on holds
+// one copy (128 KiB) until the process ends.  Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device
+// address): the address of each device's copy is looked up once per device,
+// with that device current, and kept.
+// ---------------------------------------------------------------------------
+static constexpr long long kStaticParts = 16384;   // partials: inputs of up to 16384 * 65536 = 2^30 points
+static __device__ __align__(256) double scratch[kStaticParts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (the calling thread's current device)
+int scratch_of_current(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return VB_ERR_NO_DEVICE;
+  if (dev >= kMaxDevices) return VB_ERR_PARAM;
+  std::lock_guard<std::mutex> l(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      vb::set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return VB_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return VB_OK;
+}
+}  // namespace
+
+extern "C" int vb_intf_static(const double* u, long long n, double lenx, double addition, double multiply,
+                              double* val, void* stream) {
+  if (!u || !val || n <= 0 || !(lenx > 0)) return VB_ERR_PARAM;
+  if (vb::sum_parts(n) > kStaticParts) {
+    vb::set_last_error("vb_intf_static: " + std::to_string(n) + " points exceed its maximum of " +
+                       std::to_string(kStaticParts * vb::kSumChunk));
+    return VB_ERR_PARAM;
+  }
+  double* part = nullptr;
+  const int rc = scratch_of_current(&part);
+  if (rc != VB_OK) return rc;
+  vb::launch_intf(u, n, lenx, addition, multiply, part, val, static_cast<hipStream_t>(stream));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    vb::set_last_error(std::string("vb_intf_static launch: ") + hipGetErrorString(e));
+    return VB_ERR_HIP;
+  }
+  return VB_OK;
+}

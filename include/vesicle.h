@@ -173,6 +173,20 @@ int vb_calc_etd44(vb_calc* c, void* d_real, const void* fd, const void* nlda, co
                   const void* nldc, double eps, double h);
 int vb_calc_sync(vb_calc* c);
 
+/* The integral of vb_calc_intf, (sum(u)/n + addition) * multiply * lenx^3 over
+ * n FP64 points of device memory, written to the device double *val on
+ * `stream` (a hipStream_t, NULL: the null stream) without waiting, and without
+ * allocating anything: no counterpart in the reference (its Calc::intf takes a
+ * caller scratch array, Calc.cu:255-263).  It is for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc.  The block partial sums live in a static
+ * device array of 16384 doubles per GPU (held until the process ends), so
+ * n <= 2^30 (VB_ERR_PARAM beyond); every call on a GPU shares that array, and
+ * calls that overlap on one GPU (two streams, two threads) race.  The result
+ * equals vb_calc_intf's bit for bit (the same kernels). */
+int vb_intf_static(const double* u, long long n, double lenx, double addition, double multiply, double* val,
+                   void* stream);
+
 /* ---- initial shapes (fluidbase init_* generators, seeded) ----------------- */
 /* phi = tanh(d / (sqrt2 eps)) on the reference grid x_i = i L/n - L/2
  * (fluidbase.cpp:152-161).  kind: 0 sphere, 1 ellipsoid, 2 two ellipsoids,
