@@ -192,14 +192,32 @@ def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05, sample_cap=
 
     srv = vb.Server(max_active=16, streams=4, converge_window=50, converge_tol=1e-6)
     results, t0 = {}, time.perf_counter()
+    t_note = [t0]
+
+    def progress(submitted):
+        # a line on stderr every 20 s, so that a long stream shows it is alive
+        if time.perf_counter() - t_note[0] >= 20.0:
+            t_note[0] = time.perf_counter()
+            print("serving: %.0f s, %d of %d submitted, %d done" % (t_note[0] - t0, submitted, len(reqs),
+                                                                     len(results)), file=sys.stderr, flush=True)
+
     for j, req in enumerate(reqs):
         p, phi = serve_request(vb, req, j, vb.FP32_MIXED, seed)     # prepared before its arrival time
         while time.perf_counter() - t0 < req[0]:
             for r in srv.poll():
                 results[r["id"]] = r
+            progress(j)
             time.sleep(0.0005)
         srv.submit(j, p, phi)
-    for r in srv.drain():
+    while True:
+        for r in srv.poll():
+            results[r["id"]] = r
+        st = srv.pending()
+        if st["queued"] == 0 and st["running"] == 0:
+            break
+        progress(len(reqs))
+        time.sleep(0.002)
+    for r in srv.poll():
         results[r["id"]] = r
     wall = time.perf_counter() - t0
     srv.close()
@@ -214,6 +232,8 @@ def serving(vb, seconds, rate=20.0, seed=20260005, fp64_sample=0.05, sample_cap=
     for j in sample:
         j = int(j)
         q = results[j]
+        print("serving: FP64 re-run of request %d (%d^3, %d steps)" % (j, reqs[j][1], q["steps"]),
+              file=sys.stderr, flush=True)
         p32, phi = serve_request(vb, reqs[j], j, vb.FP32_MIXED, seed)
         s32, dts = converge_run(vb, p32, phi)
         alone = s32.query()

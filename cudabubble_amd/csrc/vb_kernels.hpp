@@ -331,21 +331,34 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   // values, they replace the transform's in the stage combine
   constexpr bool DCS = sizeof(T) == 4 && !SUMS;
   double dcs[2] = {0.0, 0.0};
+  // a thread's E values are added in T first and then once in FP64: in FP32
+  // the E-term share is rounded at ~1e-7 of its size, and the shares' errors
+  // are independent, so the grid sums carry ~1e-10 relative (the per-point
+  // FP64 conversions and adds were an eighth of this kernel's instructions)
+  T ts[5] = {T(0), T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     // in the work precision: v and lap v come out of a T transform, so an
     // FP64 evaluation would add nothing to their rounding (measured: the same
-    // errors against the FP64 run, 48 fewer registers); the sums are FP64
+    // errors against the FP64 run, 48 fewer registers)
     const T v = x[e].x, lv = x[e].y;
     const T w = v * v - T(1);
     const T f = -lv * T(eps) + w * v * T(ieps);                         // f_kernel, Calc.cu:160
     const T t1 = v * (w - T(kappa));                                    // nld_kernel1, Calc.cu:270
     const T sv = T(D) * t1 + (T(3) * v * v - T(D + 1)) * (T(eps) * f);  // eps^3 s, Calc.cu:282
-    if (valid) { acc[0] += double(v); acc[1] += double(w) * double(w); acc[2] += double(f) * double(f); }
-    if constexpr (DCS) {
-      if (valid) { dcs[0] += double(t1); dcs[1] += double(sv); }
+    if constexpr (sizeof(T) == 4) {
+      ts[0] += v; ts[1] += w * w; ts[2] += f * f;
+      if constexpr (DCS) { ts[3] += t1; ts[4] += sv; }
+    } else if (valid) {
+      acc[0] += double(v); acc[1] += double(w) * double(w); acc[2] += double(f) * double(f);
     }
     x[e] = mk<T>(t1, sv);
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (valid) {
+      acc[0] = double(ts[0]); acc[1] = double(ts[1]); acc[2] = double(ts[2]);
+      dcs[0] = double(ts[3]); dcs[1] = double(ts[4]);
+    }
   }
   if constexpr (SUMS) {
     block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
@@ -1037,8 +1050,17 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
 // 0 for the row layout [z][ky][pitch], or the column tile of the z kernels'
 // group-major layout (gm_index).  One thread per mode.
 template <typename TI>
-__global__ void k_relayout(const cpx<TI>* __restrict__ in, int tb_in, cpx<double>* __restrict__ out, int tb_out,
-                           Geo g) {
+struct RelArgs {
+  const cpx<TI>* in;
+  cpx<double>* out;
+  int tb_in, tb_out;
+  Geo g;
+};
+
+template <typename TI, bool TAB = false>
+__global__ void k_relayout(RelArgs<TI> a0, const RelArgs<TI>* tab) {
+  const RelArgs<TI> a = TAB ? tab[blockIdx.z] : a0;
+  const Geo& g = a.g;
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   const long long nm = (long long)g.nz * g.ny * (g.nx / 2 + 1);
   if (i >= nm) return;
@@ -1046,8 +1068,8 @@ __global__ void k_relayout(const cpx<TI>* __restrict__ in, int tb_in, cpx<double
   const int ky = int((i / (g.nx / 2 + 1)) % g.ny);
   const int z = int(i / ((long long)(g.nx / 2 + 1) * g.ny));
   const long long rw = (long long)z * g.pp + (long long)ky * g.pitch + kx;
-  const cpx<TI> v = in[tb_in ? gm_index(g, tb_in, z, ky, kx) : rw];
-  out[tb_out ? gm_index(g, tb_out, z, ky, kx) : rw] = mk<double>(double(v.x), double(v.y));
+  const cpx<TI> v = a.in[a.tb_in ? gm_index(g, a.tb_in, z, ky, kx) : rw];
+  a.out[a.tb_out ? gm_index(g, a.tb_out, z, ky, kx) : rw] = mk<double>(double(v.x), double(v.y));
 }
 
 }  // namespace vb

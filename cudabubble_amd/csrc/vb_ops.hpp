@@ -18,7 +18,7 @@ namespace vb {
 // group, with the solves' argument structs in a device table.
 enum PipeOp {
   OP_XA_RS, OP_XA_F, OP_XA_S, OP_XA_SUMS, OP_Y_FWD, OP_Y_INV, OP_Y_FWD_GM, OP_ZLAP, OP_ZPREP,
-  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET, OP_ZS44P
+  OP_ZS41, OP_ZS42, OP_ZS43, OP_ZS44, OP_ZS21, OP_ZS22, OP_FIN, OP_KEEP, OP_RETARGET, OP_ZS44P, OP_RELAYOUT
 };
 // x-kernel outputs: the partial sums of v, (v^2-1)^2 and f^2 only (k_xrow
 // without its forward transform: the integrals of a field), tmp1 and f

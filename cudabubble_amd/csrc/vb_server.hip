@@ -4,13 +4,19 @@
 // thread owns the device.  It admits pending requests in arrival order while
 // the resident footprint fits the memory budget and a slot is free, keeps
 // every admitted solve's state, work buffers and coefficient tables resident
-// in HBM, and interleaves the solves' adaptive iterations: every running solve
-// has one attempt in flight on one of a few streams, and whichever attempt
-// finishes first is decided on the host and its next attempt issued.  A solve
-// leaves when its energy has dropped by less than a relative tolerance over a
-// window of accepted steps, when t reaches tmax, at a driver exit, or at a
-// step cap.  Requests are independent: each computes exactly what vb_step
-// computes for it alone, whatever else is running.
+// in HBM, and interleaves the solves' adaptive iterations: each attempt is
+// decided on the host when its completion event has fired.  Larger solves
+// keep one attempt each in flight on one of a few streams and are re-issued
+// as soon as they are decided.  Small solves of one kind (grid, precision,
+// scheme) run on one stream and advance in rounds: once none of a kind's
+// attempts is in flight, the next attempt of every one of them is issued as
+// shared launches, one per pipeline kernel, over all of them (grid.z runs
+// over the solves, each solve's argument struct read from a device table),
+// so that many small grids fill the GPU together.  A solve leaves when its energy
+// has dropped by less than a relative tolerance over a window of accepted
+// steps, when t reaches tmax, at a driver exit, or at a step cap.  Requests
+// are independent: each computes exactly what vb_step computes for it alone,
+// whatever else is running.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -40,12 +46,57 @@ struct Running {
   int stream;
   size_t bytes;
   double t_admit;
+  bool inflight = false;     // an attempt was issued and is not decided yet
   vb_step_report rep{};
   std::deque<double> hist;   // energies of the last window+1 accepted states
 };
 
+// the kind of a solve: solves of one kind share a stream and launches
+vb::GroupKey kind_of(const vb_params& p) {
+  vb::GroupKey k{};
+  k.ok = 1;
+  k.nx = p.nx;
+  k.nz = p.nz;
+  k.precision = p.precision;
+  k.scheme = p.scheme;
+  return k;
+}
+
+// Kinds whose solves share launches: grids of at most 64^3 points, which one
+// solve's launches cannot fill the GPU with.  Larger solves fill it alone,
+// and one launch over several of them streams several working sets through
+// the caches at once (measured in the batched form: 4 x 192^3 in shared
+// launches ran 14 % slower than one launch per solve); they interleave
+// attempt by attempt over the streams instead.
+constexpr long long kSharedMaxPoints = 64ll * 64 * 64;
+bool shared_kind(const vb_params& p) { return (long long)p.nx * p.nx * p.nz <= kSharedMaxPoints; }
+
+// a reused solver keeps its stream, which is its kind's
+bool same_kind(const vb_params& a, const vb_params& b) {
+  const vb::GroupKey x = kind_of(a), y = kind_of(b);
+  return !(x < y) && !(y < x);
+}
+
+// The argument tables of one launch sequence's shared launches (pinned
+// staging and its device copy, each launch's table at a fixed offset, room for
+// max_active solves) and its captured launches, one graph per number of
+// solves.  A kind's next round is issued only after every attempt of its
+// previous round has completed, so the tables are free when rewritten.
+struct SeqShared {
+  vb::DBuf<unsigned char> dev;
+  unsigned char* host = nullptr;
+  size_t bytes = 0;
+  std::vector<size_t> off;
+  std::map<int, hipGraphExec_t> graphs;
+  ~SeqShared() {
+    for (auto& g : graphs)
+      if (g.second) (void)hipGraphExecDestroy(g.second);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
 // A finished request's solver, kept with its memory for the next request of
-// the same grid, precision and buffer set (vb::SolverBase::reset).
+// the same kind and buffer set (vb::SolverBase::reset).
 struct Idle {
   std::unique_ptr<vb::SolverBase> s;
   int stream;
@@ -76,6 +127,8 @@ struct vb_server {
   std::map<long long, std::vector<double>> phis;
   std::vector<Running> running;   // worker thread only
   std::deque<Idle> idle;          // worker thread only: the slot pool, oldest first
+  std::map<vb::GroupKey, int> kind_stream;                           // worker thread only
+  std::map<std::vector<long long>, std::unique_ptr<SeqShared>> seqs;  // worker thread only
   size_t used = 0;                // bytes of every solver held, running or idle
   int nrun = 0;
   bool stop = false;
@@ -91,6 +144,8 @@ struct vb_server {
     if (worker.joinable()) worker.join();
     running.clear();
     idle.clear();
+    for (auto s : streams) (void)hipStreamSynchronize(s);
+    seqs.clear();
     for (auto s : streams) {
       (void)hipStreamSynchronize(s);
       (void)hipStreamDestroy(s);
@@ -151,7 +206,7 @@ struct vb_server {
         if (pending.empty() || (int)running.size() >= cfg.max_active) return;
         const vb_params& fp = pending.front().p;
         for (size_t i = 0; i < idle.size() && reuse < 0; ++i)
-          if (same_slot(idle[i].p, fp)) reuse = (int)i;
+          if (same_slot(idle[i].p, fp) && same_kind(idle[i].p, fp)) reuse = (int)i;
         if (reuse < 0) {
           bytes = vb::solver_footprint(fp);
           while (used + bytes > cfg.memory_budget && !idle.empty()) {
@@ -178,7 +233,16 @@ struct vb_server {
           r.s = std::move(it.s);
           if (!r.s->reset(q.p, q.phi0.data())) throw std::bad_alloc();   // same_slot guarantees a fit
         } else {
-          r.stream = int(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
+          // small kinds: one stream per kind, the least loaded one when the kind
+          // first appears; larger solves: the least loaded stream
+          const int least = int(std::min_element(stream_load.begin(), stream_load.end()) - stream_load.begin());
+          if (shared_kind(q.p)) {
+            auto ks = kind_stream.find(kind_of(q.p));
+            if (ks == kind_stream.end()) ks = kind_stream.emplace(kind_of(q.p), least).first;
+            r.stream = ks->second;
+          } else {
+            r.stream = least;
+          }
           r.bytes = bytes;
           r.s = vb::make_solver(q.p, streams[(size_t)r.stream]);
           r.s->set_phi(q.phi0.data());
@@ -206,9 +270,137 @@ struct vb_server {
       q.phi0.clear();
       q.phi0.shrink_to_fit();
       r.req = std::move(q);
-      running.push_back(std::move(r));
-      running.back().s->adaptive_issue(running.back().rep);
+      running.push_back(std::move(r));   // its first attempt goes out with its kind's next round
     }
+  }
+
+  // Issues the next attempt of the solves running[idx] (one kind, none in
+  // flight): alone through the solver's own graphs, or together as shared
+  // launches.  Solves whose pipelines are in different states (an accepted
+  // attempt leaves the next state's prefix done, a rejected one does not)
+  // record different launch sequences; each sequence goes out as its own
+  // shared launches.  Returns the indices of solves that left at a driver
+  // exit before their attempt.
+  std::vector<size_t> issue_round(const vb::GroupKey& k, const std::vector<size_t>& idx) {
+    std::vector<size_t> go, out;
+    for (size_t i : idx) {
+      if (running[i].s->attempt_prepare(running[i].rep)) go.push_back(i);
+      else out.push_back(i);
+    }
+    if (go.size() == 1) {
+      running[go[0]].s->issue_prepared();
+      running[go[0]].inflight = true;
+    }
+    if (go.size() < 2) return out;
+    std::vector<vb::Recorder> rec(go.size());
+    std::map<std::vector<long long>, std::vector<size_t>> parts;   // sequence -> lanes (into go)
+    for (size_t l = 0; l < go.size(); ++l) {
+      running[go[l]].s->record_attempt(rec[l]);
+      parts[signature(k, running[go[l]].s->stream(), rec[l])].push_back(l);
+    }
+    for (auto& pv : parts) {
+      const auto& lanes = pv.second;
+      const auto& s0 = rec[lanes[0]].seq;
+      hipStream_t st = running[go[lanes[0]]].s->stream();
+      SeqShared& e = seq_shared(pv.first, s0);
+      const int L = int(lanes.size());
+      for (size_t j = 0; j < s0.size(); ++j) {
+        const size_t a = s0[j].args.size();
+        for (int m = 0; m < L; ++m) {
+          const auto& src = rec[lanes[(size_t)m]].seq[j].args;
+          std::copy(src.begin(), src.end(), e.host + e.off[j] + (size_t)m * a);
+        }
+      }
+      hipGraphExec_t& gx = e.graphs[L];
+      if (!gx) {
+        // the shared launches of this sequence over L solves, captured once:
+        // the kernels read their arguments from the sequence's fixed tables
+        VB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (size_t j = 0; j < s0.size(); ++j)
+          s0[j].launch_tab(s0[j].op, e.dev.p + e.off[j], s0[j].grid, s0[j].block, L, st);
+        hipGraph_t gr = nullptr;
+        VB_HIP(hipStreamEndCapture(st, &gr));
+        const hipError_t err = hipGraphInstantiateWithFlags(&gx, gr, 0);
+        (void)hipGraphDestroy(gr);
+        if (err != hipSuccess) {
+          gx = nullptr;
+          throw vb::HipError{err, "hipGraphInstantiateWithFlags"};
+        }
+      }
+      VB_HIP(hipMemcpyAsync(e.dev.p, e.host, e.bytes, hipMemcpyHostToDevice, st));
+      VB_HIP(hipGraphLaunch(gx, st));
+    }
+    for (size_t l = 0; l < go.size(); ++l) {
+      running[go[l]].s->attempt_issued();
+      running[go[l]].inflight = true;
+    }
+    return out;
+  }
+
+  // the launch sequence of a recorded attempt, with its kind and stream
+  static std::vector<long long> signature(const vb::GroupKey& k, hipStream_t st, const vb::Recorder& r) {
+    std::vector<long long> v{k.nx, k.nz, k.precision, k.scheme, (long long)(intptr_t)st};
+    for (const auto& x : r.seq) {
+      v.push_back(x.op);
+      v.push_back((long long)(intptr_t)x.launch_tab);
+      v.push_back((long long)x.args.size());
+      v.push_back(x.grid.x);
+      v.push_back(x.grid.y);
+      v.push_back(x.block.x);
+    }
+    return v;
+  }
+
+  // the tables and captured graphs of one launch sequence, sized for
+  // max_active solves when the sequence first appears
+  SeqShared& seq_shared(const std::vector<long long>& sig, const std::vector<vb::RecLaunch>& seq) {
+    auto& e = seqs[sig];
+    if (!e) {
+      e = std::make_unique<SeqShared>();
+      e->off.resize(seq.size());
+      size_t bytes = 0;
+      for (size_t j = 0; j < seq.size(); ++j) {
+        e->off[j] = bytes;
+        bytes += (seq[j].args.size() * (size_t)cfg.max_active + 255) / 256 * 256;
+      }
+      e->bytes = bytes;
+      VB_HIP(hipHostMalloc((void**)&e->host, bytes, hipHostMallocDefault));
+      e->dev.alloc(bytes);
+    }
+    return *e;
+  }
+
+  // the next attempt of every solve that has none in flight: larger solves at
+  // once, each alone; small kinds in rounds, once none of the kind's solves
+  // has an attempt in flight.  True if anything was issued or left.
+  bool issue_rounds() {
+    std::map<vb::GroupKey, std::vector<size_t>> ready;
+    std::map<vb::GroupKey, bool> busy;
+    std::vector<size_t> gone;
+    bool any = false;
+    for (size_t i = 0; i < running.size(); ++i) {
+      const vb::GroupKey k = kind_of(running[i].req.p);
+      if (!shared_kind(running[i].req.p)) {
+        if (running[i].inflight) continue;
+        any = true;
+        for (size_t j : issue_round(k, {i})) gone.push_back(j);
+      } else if (running[i].inflight) {
+        busy[k] = true;
+      } else {
+        ready[k].push_back(i);
+      }
+    }
+    for (auto& kv : ready) {
+      if (busy.count(kv.first)) continue;
+      any = true;
+      for (size_t i : issue_round(kv.first, kv.second)) gone.push_back(i);
+    }
+    std::sort(gone.begin(), gone.end());
+    for (size_t m = gone.size(); m-- > 0;) {
+      finish(running[gone[m]]);
+      running.erase(running.begin() + (long)gone[m]);
+    }
+    return any;
   }
 
   void loop() {
@@ -224,20 +416,21 @@ struct vb_server {
         bool progressed = false;
         for (size_t i = 0; i < running.size();) {
           Running& r = running[i];
-          if (!r.s->attempt_ready()) {
+          if (!r.inflight || !r.s->attempt_ready()) {
             ++i;
             continue;
           }
           progressed = true;
+          r.inflight = false;
           const int ended = r.s->adaptive_complete(r.rep);
           if (ended && converged(r)) {
             finish(r);
             running.erase(running.begin() + (long)i);
             continue;
           }
-          r.s->adaptive_issue(r.rep);
           ++i;
         }
+        if (issue_rounds()) progressed = true;
         if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
       }
     } catch (const vb::HipError& e) {

@@ -95,6 +95,12 @@ __global__ void k_fill_scal(Scal* sc, double alpha, double beta, double alpha0, 
   }
 }
 
+template <typename TI>
+static void relayout_tab(int, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
+  gr.z = unsigned(lanes);
+  hipLaunchKernelGGL((k_relayout<TI, true>), gr, bl, 0, s, RelArgs<TI>{}, (const RelArgs<TI>*)tab);
+}
+
 static void finalize_tab(int, const void* tab, dim3 gr, dim3 bl, int lanes, hipStream_t s) {
   gr.z = unsigned(lanes);
   hipLaunchKernelGGL(k_finalize<true>, gr, bl, 0, s, FinArgs{}, (const FinArgs*)tab);
@@ -476,7 +482,10 @@ struct Solver final : SolverBase {
   template <typename TI>
   void relayout(const cpx<TI>* in, int tb_in, cpx<double>* out, int tb_out) {
     const long long nm = (long long)g.nz * g.ny * (g.nx / 2 + 1);
-    hipLaunchKernelGGL(k_relayout<TI>, dim3(unsigned((nm + 255) / 256)), dim3(256), 0, st, in, tb_in, out, tb_out, g);
+    const RelArgs<TI> a{in, out, tb_in, tb_out, g};
+    const dim3 gr(unsigned((nm + 255) / 256)), bl(256);
+    if (g_recorder) return record_launch(&relayout_tab<TI>, OP_RELAYOUT, a, gr, bl);
+    hipLaunchKernelGGL((k_relayout<TI, false>), gr, bl, 0, st, a, nullptr);
   }
   void check_launch() { VB_HIP(hipGetLastError()); }
 
@@ -860,6 +869,10 @@ struct Solver final : SolverBase {
   static constexpr bool kMixed = sizeof(T) == 4;
 
   void adaptive_issue(vb_step_report& r) override {
+    if (attempt_prepare(r)) issue_prepared();
+  }
+
+  bool attempt_prepare(vb_step_report& r) override {
     hs_state = false;
     if (!started) start();
     if (kMixed) need64();   // allocations stay outside graph capture
@@ -884,10 +897,14 @@ struct Solver final : SolverBase {
     }
     if (time_step < 1e-23) {
       r.status = VB_ERR_DT_UNDERFLOW;
-      return;
+      return false;
     }
     if (!att_ev) VB_HIP(hipEventCreateWithFlags(&att_ev, hipEventDisableTiming));
     att_keep = cur;
+    return true;
+  }
+
+  void issue_prepared() override {
     if (p.graphs && !prof) {
       // one graph per pipeline state and dt; the ladder of dt values recurs
       StepGraph* G = find_graph(kAttempt);
@@ -897,6 +914,24 @@ struct Solver final : SolverBase {
     } else {
       attempt_body();
     }
+    attempt_issued();
+  }
+
+  void record_attempt(Recorder& rec) override {
+    (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // the table exists before recording
+    const PipeState pre = pipe_state();
+    g_recorder = &rec;
+    try {
+      attempt_body();
+    } catch (...) {
+      g_recorder = nullptr;
+      set_pipe_state(pre);
+      throw;
+    }
+    g_recorder = nullptr;
+  }
+
+  void attempt_issued() override {
     VB_HIP(hipMemcpyAsync(hs, kMixed ? scal64.p : scal.p, sizeof(Scal), hipMemcpyDeviceToHost, st));
     VB_HIP(hipEventRecord(att_ev, st));
     att_pending = true;

@@ -68,6 +68,18 @@ struct SolverBase {
   virtual void adaptive_issue(vb_step_report& rep) = 0;
   virtual bool attempt_ready() = 0;
   virtual int adaptive_complete(vb_step_report& rep) = 0;
+  // adaptive_issue() in its parts, so that a scheduler can issue the attempts
+  // of several solves as shared launches: attempt_prepare() is the host half
+  // (the iteration's dt, the current energy; false after a driver exit in
+  // rep.status), then either issue_prepared() launches the attempt itself, or
+  // record_attempt() records its launches instead (its tables uploaded on
+  // stream() first; the host state advances as if they ran), the caller
+  // issues them on stream(), and attempt_issued() adds the copy of the
+  // candidate's scalars and the completion event.
+  virtual bool attempt_prepare(vb_step_report& rep) = 0;
+  virtual void issue_prepared() = 0;
+  virtual void record_attempt(Recorder& rec) = 0;
+  virtual void attempt_issued() = 0;
   // The request loop's slot pool: reuse this solver's memory for another solve
   // of the same grid, precision and buffer set (false if p needs others).
   // Captured graphs and tables of the old parameters are dropped; phi0 is

@@ -4,6 +4,7 @@
     python -m cudabubble_amd.rocprof_summary traffic --fetch F.csv --write W.csv \
         --cal-fetch CF.csv --cal-write CW.csv --n 256 --cmd "..." [--json-dir DIR --json-prec fp32]
     python -m cudabubble_amd.rocprof_summary sequence KERNEL_TRACE.csv --n 256 --prec fp32 --cmd "..."
+    python -m cudabubble_amd.rocprof_summary counters A.csv B.csv --n 256 --cmd "..." --cmd "..."
 
 `stats` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown
 table keyed by the solver's own kernel names (the names `vb_profile` uses).
@@ -15,6 +16,8 @@ known size (the `bench.py --copy-calibration` run), and optionally writes one
 solver kernels of one size and precision, the average duration of each kernel
 by the kernel that ran just before it, and the idle time between consecutive
 dispatches.
+`counters` joins `--pmc` runs of SQ and GRBM counters into instructions per
+wave and the VALU issue fraction of each kernel of one length.
 """
 import argparse
 import collections
@@ -160,6 +163,65 @@ def sequence(path, n, prec, cmd):
     return "\n".join(out) + "\n"
 
 
+N_SIMD = 1024      # MI355X: 256 CUs x 4 SIMDs
+N_XCD = 8          # GRBM_GUI_ACTIVE is summed over the 8 XCDs
+
+
+def _pmc_means(paths):
+    """{(kernel name, counter): mean per dispatch} over the dispatches of each kernel."""
+    acc = collections.defaultdict(list)
+    for p in paths:
+        for r in csv.DictReader(open(p)):
+            acc[(r["Kernel_Name"], r["Counter_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+
+def counters(paths, n, cmds):
+    """Per kernel of length n, both precisions: instructions per wave by kind,
+    the VALU issue fraction and the LDS bank-conflict share.  A wave64 VALU
+    instruction (FP32 unpacked or FP64) holds its 16-lane SIMD for 4 cycles, so
+    4 SQ_INSTS_VALU / (SIMDs x kernel cycles) is the fraction of the SIMDs'
+    cycles spent issuing VALU work; kernel cycles are GRBM_GUI_ACTIVE / XCDs."""
+    m, cnt = _pmc_means(paths)
+    kernels = sorted({k for k, _ in m})
+    rows = []
+    for k in kernels:
+        sn = solver_name(k)
+        if not sn or sn[1] is None or kernel_size(k) != str(n):
+            continue
+        g = lambda c: m.get((k, c))
+        waves, valu, grbm = g("SQ_WAVES"), g("SQ_INSTS_VALU"), g("GRBM_GUI_ACTIVE")
+        if not waves or valu is None:
+            continue
+        cyc = grbm / N_XCD if grbm else None
+        per = lambda c: (g(c) / waves) if g(c) is not None else None
+        rows.append({
+            "kernel": sn[0], "prec": sn[1], "launches": cnt.get((k, "SQ_WAVES"), 0), "waves": waves,
+            "cycles": cyc, "valu_pw": valu / waves, "lds_pw": per("SQ_INSTS_LDS"),
+            "vmrd_pw": per("SQ_INSTS_VMEM_RD"), "vmwr_pw": per("SQ_INSTS_VMEM_WR"),
+            "valu_frac": 4 * valu / (N_SIMD * cyc) if cyc else None,
+            "conf": g("SQ_LDS_BANK_CONFLICT"),
+            "wait": (g("SQ_WAIT_INST_ANY") / g("SQ_WAVE_CYCLES")) if g("SQ_WAIT_INST_ANY") and g("SQ_WAVE_CYCLES") else None,
+            "name": k})
+    f = lambda x, s: ("-" if x is None else s % x)
+    out = ["# Instruction mix and VALU issue per launch, rocprofv3 --pmc (%d^3)" % n, "",
+           "Commands (repository root):", ""] + ["    " + c for c in cmds] + [
+           "", "Means over each kernel's launches.  VALU issue = 4 SQ_INSTS_VALU / (%d SIMDs x "
+           "GRBM_GUI_ACTIVE / %d XCDs): the fraction of SIMD cycles that issued VALU work (a wave64 "
+           "VALU instruction, FP32 unpacked or FP64, holds a 16-lane SIMD for 4 cycles).  Wait = "
+           "SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES.  LDS conflict = SQ_LDS_BANK_CONFLICT (cycles, summed "
+           "over CUs) per launch." % (N_SIMD, N_XCD), "",
+           "| kernel | precision | waves | kernel cycles | VALU / wave | LDS / wave | VMEM rd / wave | "
+           "VMEM wr / wave | VALU issue | wait | LDS conflict |",
+           "|---|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for r in sorted(rows, key=lambda r: (r["kernel"], r["prec"])):
+        out.append("| %s | %s | %d | %s | %.0f | %s | %s | %s | %s | %s | %s |" % (
+            r["kernel"], r["prec"], r["waves"], f(r["cycles"], "%.0f"), r["valu_pw"], f(r["lds_pw"], "%.0f"),
+            f(r["vmrd_pw"], "%.0f"), f(r["vmwr_pw"], "%.0f"), f(r["valu_frac"], "%.2f"), f(r["wait"], "%.2f"),
+            f(r["conf"], "%.3g")))
+    return "\n".join(out) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -178,8 +240,14 @@ def main(argv=None):
     q.add_argument("--n", type=int, required=True)
     q.add_argument("--prec", default="fp32")
     q.add_argument("--cmd", required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--n", type=int, required=True)
+    c.add_argument("--cmd", action="append", required=True)
     a = ap.parse_args(argv)
-    if a.what == "sequence":
+    if a.what == "counters":
+        sys.stdout.write(counters(a.csv, a.n, a.cmd))
+    elif a.what == "sequence":
         sys.stdout.write(sequence(a.csv, a.n, a.prec, a.cmd))
     elif a.what == "stats":
         sys.stdout.write(stats(a.csv, a.cmd))

@@ -108,3 +108,28 @@ def test_slot_pool_reuse_matches_fresh_solves(vb, gpu):
         assert np.array_equal(s.get_phi(), phis[i]), i
         assert s.query() == (res[i]["E"], res[i]["V"], res[i]["A"], res[i]["Eb"]), i
         s.close()
+
+
+def test_shared_launch_rounds_match_solo_solves(vb, gpu):
+    # seven requests of one kind (32^3 FP32-mixed ETDRK4) and three of another
+    # (32^3 FP64): the attempts of a kind's running solves go out together as
+    # shared launches, each solve with its own dt, tables, constants and state;
+    # two 96^3 requests beside them go out one attempt at a time; every result
+    # is what a fresh vb_step computes for that request alone
+    reqs = [request(vb, 32, 1 + k % 4, 70 + k, 1) for k in range(7)] + \
+           [request(vb, 32, 1 + k % 4, 80 + k, 0) for k in range(3)] + \
+           [request(vb, 96, 1 + k, 90 + k, 1) for k in range(2)]
+    for k, (p, _) in enumerate(reqs):
+        p.M1 = 1e3 * (1 + k)
+        p.eps *= 1.0 + 0.1 * (k % 3)
+    flush_print("server: shared rounds")
+    res, phis = run(vb, reqs, max_active=16, streams=4, max_steps=15)
+    for i, (p, phi) in enumerate(reqs):
+        s = vb.Solver(p, phi)
+        r = s.step(15)
+        assert res[i]["status"] == 0 and res[i]["steps"] == 15, res[i]
+        assert (r["steps"], r["evaluations"], r["rejections"]) == \
+            (res[i]["steps"], res[i]["evaluations"], res[i]["rejections"]), i
+        assert np.array_equal(s.get_phi(), phis[i]), i
+        assert s.query() == (res[i]["E"], res[i]["V"], res[i]["A"], res[i]["Eb"]), i
+        s.close()

@@ -55,3 +55,25 @@ def test_traffic_scaling_and_json_choice(tmp_path):
     # the most-launched instantiation at length 256, not the 64^3 one with more launches
     assert j["rocprof_name"] == main and j["launches"] == 5
     assert j["read_bytes_per_launch"] == 2000.0 * 1024 and j["write_bytes_per_launch"] == 500.0 * 1024
+
+
+def test_counters_valu_issue(tmp_path):
+    k32 = "void vb::k_xrow<float, 256, false, 1, false>(vb::XaArgs<float>, vb::XaArgs<float> const*)"
+    k64 = "void vb::k_xrow<double, 256, false, 1, false>(vb::XaArgs<double>, vb::XaArgs<double> const*)"
+    with open(tmp_path / "a.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        for k, waves, valu in ((k32, 16384, 2.6e7), (k64, 16384, 2.4e7)):
+            for _ in range(3):
+                w.writerow({"Kernel_Name": k, "Counter_Name": "SQ_WAVES", "Counter_Value": waves})
+                w.writerow({"Kernel_Name": k, "Counter_Name": "SQ_INSTS_VALU", "Counter_Value": valu})
+    with open(tmp_path / "b.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value"])
+        w.writeheader()
+        w.writerow({"Kernel_Name": k32, "Counter_Name": "GRBM_GUI_ACTIVE", "Counter_Value": 8 * 1.25e5})
+    md = rs.counters([str(tmp_path / "a.csv"), str(tmp_path / "b.csv")], 256, ["cmd"])
+    # 4 x 2.6e7 / (1024 SIMDs x 1.25e5 cycles) = 0.81
+    row32 = [l for l in md.splitlines() if l.startswith("| xa | fp32 |")][0]
+    assert "| 0.81 |" in row32 and "| 1587 |" in row32
+    row64 = [l for l in md.splitlines() if l.startswith("| xa | fp64 |")][0]
+    assert "| - |" in row64   # no cycles counted for it
