@@ -137,6 +137,7 @@ def lib():
     L.vb_batch_report.argtypes = [P, I, C.POINTER(StepReport)]
     L.vb_batch_query.argtypes = [P, I, C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(D)]
     L.vb_batch_get_phi.argtypes = [P, I, P]
+    L.vb_batch_set_shared_stages.argtypes = [P, I, C.POINTER(I)]
     L.vb_batch_destroy.argtypes = [P]
     L.vb_server_config_default.argtypes = [C.POINTER(ServerConfig)]
     L.vb_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(P)]
@@ -363,6 +364,13 @@ class Batch:
         r = StepReport()
         _check(lib().vb_batch_report(self.h, i, C.byref(r)), "vb_batch_report")
         return r.as_dict()
+
+    def set_shared_stages(self, on=True):
+        """vb_batch_set_shared_stages: the ETD stage combines of the size groups
+        of several lengths as one launch each; returns how many groups share."""
+        m = C.c_int()
+        _check(lib().vb_batch_set_shared_stages(self.h, int(on), C.byref(m)), "vb_batch_set_shared_stages")
+        return m.value
 
     def query(self, i):
         E, V, A, Eb = C.c_double(), C.c_double(), C.c_double(), C.c_double()

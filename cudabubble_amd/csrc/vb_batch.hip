@@ -15,12 +15,14 @@
 // iteration; they step after the fixed-dt work has been issued and overlap
 // with it.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <numeric>
 #include <vector>
 
 #include "vb_host.hpp"
+#include "vb_multi.hpp"
 #include "vb_solver.hpp"
 
 namespace {
@@ -43,6 +45,26 @@ struct Group {
     if (gx) (void)hipGraphExecDestroy(gx);
   }
 };
+
+// Size groups of several grid lengths whose ETD stage combines share
+// launches (vb_batch_set_shared_stages): each group's other kernels stay one
+// launch per group on the group's stream, and each stage combine is one launch
+// over the solves of every member (vb_multi.hip), the member streams joined
+// before it and forked after it, all in one graph of two iterations.
+struct Shared {
+  std::vector<Group*> members;
+  hipGraphExec_t gx = nullptr;
+  std::vector<std::unique_ptr<vb::DBuf<unsigned char>>> tabs;
+  std::vector<hipEvent_t> events;
+  void clear() {
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+    tabs.clear();
+    for (auto e : events) (void)hipEventDestroy(e);
+    events.clear();
+  }
+  ~Shared() { clear(); }
+};
 }  // namespace
 
 struct vb_batch {
@@ -50,7 +72,10 @@ struct vb_batch {
   std::vector<std::unique_ptr<Group>> groups;   // a solve that cannot be grouped is a group of one
   std::vector<hipStream_t> streams;
   std::vector<vb_step_report> reports;    // of the last vb_batch_step
+  Shared shared;                          // empty unless shared stage launches are on
   ~vb_batch() {
+    for (auto s : streams) (void)hipStreamSynchronize(s);
+    shared.clear();
     groups.clear();
     solves.clear();
     for (auto s : streams) {
@@ -105,6 +130,184 @@ struct vb_batch {
       throw vb::HipError{e, "hipGraphInstantiateWithFlags"};
     }
     return true;
+  }
+
+  // The member groups of the shared stage launches: fixed-dt FP32-mixed
+  // ETDRK4 groups whose length a shared launch carries, with the default
+  // column tile, of the drift mode with the most points; none unless there
+  // are at least two such groups.
+  void choose_shared() {
+    shared.clear();
+    shared.members.clear();
+    std::map<int, std::vector<Group*>> by_drift;
+    std::map<int, long long> pts;
+    for (auto& g : groups) {
+      const vb::SolverBase& s0 = *solves[(size_t)g->lanes[0]];
+      const vb::GroupKey k = s0.group_key();
+      if (!k.ok || k.precision != VB_FP32_MIXED || k.scheme != VB_ETDRK4 || k.nx != k.nz ||
+          !vb::zstage_multi_size(k.nx, s0.z_tile()))
+        continue;
+      by_drift[k.drift].push_back(g.get());
+      pts[k.drift] += (long long)g->lanes.size() * k.nx * k.nx * k.nz;
+    }
+    int best = -1;
+    for (auto& kv : by_drift)
+      if (kv.second.size() >= 2 && (best < 0 || pts[kv.first] > pts[best])) best = kv.first;
+    if (best >= 0) shared.members = by_drift[best];
+  }
+
+  bool is_member(const Group* g) const {
+    return std::find(shared.members.begin(), shared.members.end(), g) != shared.members.end();
+  }
+
+  hipEvent_t new_event() {
+    hipEvent_t e = nullptr;
+    VB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    shared.events.push_back(e);
+    return e;
+  }
+
+  // Records two iterations of every lane of every member and captures them:
+  // per-group table launches on the groups' streams, one shared launch per
+  // stage combine.  False if a lane is not in its recurring state yet or the
+  // launch sequences do not line up.
+  bool build_shared() {
+    shared.clear();
+    auto& M = shared.members;
+    std::vector<std::vector<vb::Recorder>> rec(M.size());
+    for (size_t m = 0; m < M.size(); ++m) {
+      rec[m].resize(M[m]->lanes.size());
+      for (size_t l = 0; l < rec[m].size(); ++l)
+        if (solves[(size_t)M[m]->lanes[l]]->record_pair(rec[m][l]) == 0) return false;
+    }
+    const auto& s0 = rec[0][0].seq;
+    for (size_t m = 0; m < M.size(); ++m) {
+      const auto& g0 = rec[m][0].seq;
+      for (size_t l = 0; l < rec[m].size(); ++l) {
+        const auto& sl = rec[m][l].seq;
+        if (sl.size() != s0.size()) return false;
+        for (size_t k = 0; k < s0.size(); ++k) {
+          if (sl[k].op != s0[k].op) return false;   // every lane of every member: the same ops in order
+          if (sl[k].launch_tab != g0[k].launch_tab || sl[k].args.size() != g0[k].args.size() ||
+              sl[k].grid.x != g0[k].grid.x || sl[k].grid.y != g0[k].grid.y || sl[k].block.x != g0[k].block.x)
+            return false;   // within a group: the same launches
+        }
+      }
+    }
+    auto upload = [&](const std::vector<unsigned char>& h) {
+      auto t = std::make_unique<vb::DBuf<unsigned char>>();
+      t->alloc(h.size());
+      VB_HIP(hipMemcpy(t->p, h.data(), h.size(), hipMemcpyHostToDevice));
+      shared.tabs.push_back(std::move(t));
+      return shared.tabs.back()->p;
+    };
+    // the tables: per member and launch for the group launches; per stage
+    // combine, the argument structs, first column groups and lengths of all
+    // solves of all members
+    std::vector<std::vector<unsigned char*>> gtab(M.size(), std::vector<unsigned char*>(s0.size(), nullptr));
+    std::vector<vb::ZsMultiArgs> margs(s0.size());
+    std::vector<unsigned> mblocks(s0.size(), 0);
+    for (size_t k = 0; k < s0.size(); ++k) {
+      if (vb::zstage_multi_op(s0[k].op)) {
+        std::vector<unsigned char> args;
+        std::vector<int> first, len;
+        int acc = 0;
+        for (size_t m = 0; m < M.size(); ++m)
+          for (size_t l = 0; l < rec[m].size(); ++l) {
+            const auto& x = rec[m][l].seq[k];
+            if (x.block.x != 256 || x.grid.y != 1) return false;
+            args.insert(args.end(), x.args.begin(), x.args.end());
+            first.push_back(acc);
+            len.push_back(solves[(size_t)M[m]->lanes[l]]->p.nx);
+            acc += (int)x.grid.x;
+          }
+        std::vector<unsigned char> hf(first.size() * sizeof(int)), hn(len.size() * sizeof(int));
+        std::memcpy(hf.data(), first.data(), hf.size());
+        std::memcpy(hn.data(), len.data(), hn.size());
+        margs[k].args = upload(args);
+        margs[k].first = reinterpret_cast<const int*>(upload(hf));
+        margs[k].n = reinterpret_cast<const int*>(upload(hn));
+        margs[k].nsolve = (int)first.size();
+        mblocks[k] = (unsigned)acc;
+      } else {
+        for (size_t m = 0; m < M.size(); ++m) {
+          const size_t a = rec[m][0].seq[k].args.size();
+          std::vector<unsigned char> h(a * rec[m].size());
+          for (size_t l = 0; l < rec[m].size(); ++l)
+            std::copy(rec[m][l].seq[k].args.begin(), rec[m][l].seq[k].args.end(), h.begin() + l * a);
+          gtab[m][k] = upload(h);
+        }
+      }
+    }
+    hipStream_t main = streams[(size_t)M[0]->stream];
+    std::vector<hipStream_t> others;
+    for (auto* g : M) {
+      hipStream_t s = streams[(size_t)g->stream];
+      if (s != main && std::find(others.begin(), others.end(), s) == others.end()) others.push_back(s);
+    }
+    auto join = [&] {
+      for (hipStream_t s : others) {
+        hipEvent_t e = new_event();
+        VB_HIP(hipEventRecord(e, s));
+        VB_HIP(hipStreamWaitEvent(main, e, 0));
+      }
+    };
+    auto fork = [&] {
+      hipEvent_t e = new_event();
+      VB_HIP(hipEventRecord(e, main));
+      for (hipStream_t s : others) VB_HIP(hipStreamWaitEvent(s, e, 0));
+    };
+    VB_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+    hipGraph_t gr = nullptr;
+    try {
+      fork();
+      for (size_t k = 0; k < s0.size(); ++k) {
+        if (vb::zstage_multi_op(s0[k].op)) {
+          join();
+          vb::launch_zstage_multi(s0[k].op, margs[k], mblocks[k], main);
+          fork();
+        } else {
+          for (size_t m = 0; m < M.size(); ++m) {
+            const auto& x = rec[m][0].seq[k];
+            x.launch_tab(x.op, gtab[m][k], x.grid, x.block, (int)rec[m].size(), streams[(size_t)M[m]->stream]);
+          }
+        }
+      }
+      join();
+    } catch (...) {
+      (void)hipStreamEndCapture(main, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      throw;
+    }
+    VB_HIP(hipStreamEndCapture(main, &gr));
+    const hipError_t e = hipGraphInstantiateWithFlags(&shared.gx, gr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      shared.gx = nullptr;
+      throw vb::HipError{e, "hipGraphInstantiateWithFlags"};
+    }
+    return true;
+  }
+
+  // up to `c` more iterations of every lane of the shared members
+  int advance_shared(int c) {
+    bool ok = true, rec = shared.gx != nullptr;
+    for (auto* g : shared.members)
+      for (int i : g->lanes) {
+        ok = ok && reports[(size_t)i].status == VB_OK;
+        rec = rec && solves[(size_t)i]->at_recorded_state();
+      }
+    if (ok && c >= 2 && (rec || build_shared())) {
+      VB_HIP(hipGraphLaunch(shared.gx, streams[(size_t)shared.members[0]->stream]));
+      for (auto* g : shared.members)
+        for (int i : g->lanes) solves[(size_t)i]->advance_recorded(&reports[(size_t)i]);
+      return 2;
+    }
+    // lanes not yet in their recurring state: per-solve launches
+    for (auto* g : shared.members)
+      for (int i : g->lanes)
+        if (reports[(size_t)i].status == VB_OK) solves[(size_t)i]->launch_steps(1, &reports[(size_t)i]);
+    return 1;
   }
 
   bool ready(const Group& g) const {
@@ -231,9 +434,13 @@ int vb_batch_step(vb_batch* b, int n_steps) {
     // filling one stream's queue while the others wait.
     constexpr int chunk = 16;
     const size_t ns = b->streams.size();
+    // the members of the shared stage launches go first, all their steps
+    if (!b->shared.members.empty())
+      for (int done = 0; done < n_steps;) done += b->advance_shared(n_steps - done);
     std::vector<std::vector<Group*>> lists(ns);
     for (auto& g : b->groups)
-      if (b->solves[(size_t)g->lanes[0]]->fixed_dt()) lists[(size_t)g->stream].push_back(g.get());
+      if (b->solves[(size_t)g->lanes[0]]->fixed_dt() && !b->is_member(g.get()))
+        lists[(size_t)g->stream].push_back(g.get());
     std::vector<size_t> cur(ns, 0);
     std::vector<int> done(ns, 0);
     for (bool any = true; any;) {
@@ -278,6 +485,21 @@ int vb_batch_get_phi(vb_batch* b, int i, double* host_out) {
   if (!b || i < 0 || i >= (int)b->solves.size() || !host_out) return VB_ERR_PARAM;
   return bguard([&] {
     b->solves[(size_t)i]->get_phi(host_out);
+    return (int)VB_OK;
+  });
+}
+
+int vb_batch_set_shared_stages(vb_batch* b, int on, int* members) {
+  if (!b) return VB_ERR_PARAM;
+  return bguard([&] {
+    for (auto s : b->streams) VB_HIP(hipStreamSynchronize(s));
+    if (on) {
+      b->choose_shared();
+    } else {
+      b->shared.clear();
+      b->shared.members.clear();
+    }
+    if (members) *members = (int)b->shared.members.size();
     return (int)VB_OK;
   });
 }

@@ -72,10 +72,14 @@ __device__ __forceinline__ cpx<T> pack2(const cpx<T>* __restrict__ A, const cpx<
 }
 
 // Layout of half-spectrum rows: row r = z * ny + y starts at z * pp + y * pitch.
+// Rows number at most 512 * 512, so the division is a 32-bit one.
 struct Rows {
   int ny, pitch;
   long long pp;
-  __host__ __device__ long long off(long long r) const { return (r / ny) * pp + (r % ny) * pitch; }
+  __host__ __device__ long long off(long long r) const {
+    const unsigned q = unsigned(r) / unsigned(ny), m = unsigned(r) - q * unsigned(ny);
+    return (long long)q * pp + (long long)m * pitch;
+  }
 };
 
 // After a forward transform of Z = a + i b (a, b real rows), write the half
@@ -481,7 +485,8 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
 // ---------------------------------------------------------------------------
 // Column of one y transform: (z, kx) -> base address; rows are pitch apart.
 struct YIdx {
-  long long base;
+  long long base;   // z * pp + kx
+  int z, kx;
   bool valid;
 };
 
@@ -516,6 +521,8 @@ __device__ __forceinline__ YIdx yidx(const Geo& g, long long t) {
     kx = int(b % g.nh);
   }
   r.base = (long long)z * g.pp + kx;
+  r.z = z;
+  r.kx = kx;
   return r;
 }
 template <int TB>
@@ -540,20 +547,22 @@ struct YArgs {
 // stage kernels: [z workgroup][kz][ztb], the workgroup and lane that zidx
 // gives to column (ky, kx).
 __device__ __forceinline__ long long gm_index(const Geo& g, int ztb, int z, int ky, int kx) {
+  // ztb is a power of two (every column tile is; the solver checks it)
+  const int lz = __builtin_ctz((unsigned)ztb), mz = ztb - 1;
   long long grp;
   int c;
   if (g.aligned) {
     if (kx == g.nx / 2) {
-      grp = ky / ztb;
-      c = ky % ztb;
+      grp = ky >> lz;
+      c = ky & mz;
     } else {
-      grp = (long long)((g.ny + ztb - 1) / ztb) + (long long)ky * ((g.nx / 2) / ztb) + kx / ztb;
-      c = kx % ztb;
+      grp = (long long)((g.ny + mz) >> lz) + (long long)ky * ((g.nx / 2) >> lz) + (kx >> lz);
+      c = kx & mz;
     }
   } else {
     const long long q = (long long)ky * g.nh + kx;
-    grp = q / ztb;
-    c = int(q % ztb);
+    grp = q >> lz;
+    c = int(q & mz);
   }
   return (grp * g.nz + z) * ztb + c;
 }
@@ -593,9 +602,16 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
   if (id.valid) {
     if constexpr (GM) {
       cpx<T>* out = blockIdx.y ? a.o1 : a.o0;
-      const int z = int(id.base / g.pp), kx = int(id.base % g.pp);
+      if (g.aligned && id.kx != g.nx / 2) {
+        // ky = j + TF e: one ky step is one row of column groups of all z
+        cpx<T>* o = out + gm_index(g, a.ztb, id.z, j, id.kx);
+        const long long s = (long long)(((g.nx / 2) >> __builtin_ctz((unsigned)a.ztb)) * TF) * g.nz * a.ztb;
 #pragma unroll
-      for (int e = 0; e < E; ++e) out[gm_index(g, a.ztb, z, j + TF * e, kx)] = x[e];
+        for (int e = 0; e < E; ++e) o[e * s] = x[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) out[gm_index(g, a.ztb, id.z, j + TF * e, id.kx)] = x[e];
+      }
     } else {
 #pragma unroll
       for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
@@ -769,7 +785,7 @@ __device__ __forceinline__ void next_field_elem(double re, double im, int kz, co
 template <typename T, int N, int TB, int V, bool PONLY = false, class Lds>
 __device__ __forceinline__ void next_field_tail(cpx<T>* x, cpx<T>* y, double acc, int j, const ZIdx& id,
                                                 const Geo& g, Lds& lds, Lds& lds2, const cpx<T>* __restrict__ tw,
-                                                cpx<T>* P, cpx<T>* Q, double* part, double* red) {
+                                                cpx<T>* P, cpx<T>* Q, double* part, double* red, int tile) {
   constexpr int E = Plan<N, V>::E, TF = N / E;
   const long long plane = g.pp;
   if constexpr (PONLY)
@@ -784,7 +800,7 @@ __device__ __forceinline__ void next_field_tail(cpx<T>* x, cpx<T>* y, double acc
     }
   }
   double s[1] = {acc};
-  block_sum_store<1, TB * TF>(s, red, part + blockIdx.x);
+  block_sum_store<1, TB * TF>(s, red, part + tile);
 }
 
 template <typename T>
@@ -821,7 +837,7 @@ __global__ void __launch_bounds__((TB * TFof<N>())) k_zprep(ZprArgs<T> a0, const
   double acc = 0.0;
 #pragma unroll
   for (int e = 0; e < E; ++e) next_field_elem<T>(u[e].x, u[e].y, j + TF * e, id, g, acc, x[e], y[e]);
-  next_field_tail<T, N, TB, 0>(x, y, acc, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB, 0>(x, y, acc, j, id, g, lds, lds2, tw, a.P, a.Q, a.part, red, int(blockIdx.x));
 }
 
 template <typename T>
@@ -869,11 +885,22 @@ struct ZsArgs {
 // 182 us), and so did 4 x 4 x 4 x 4 over 64 threads (100-128 registers).
 template <typename T, int N> constexpr int zsplan() { return 0; }
 
-template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
-          bool PROBE = false>
-__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
+// LDS bytes of a stage combine: two column tiles for the transforms, or the
+// staged coefficient table, whichever is larger
+template <typename T, int N, int SCH, int STG, int TB, int V>
+constexpr int zs_lds_bytes() {
+  return (2 * N * TB * int(sizeof(cpx<T>))) >
+                 ((SCH == 4 && STG != 2) ? TB * (N / 2 + 1) * int(STG == 4 ? sizeof(double4) : sizeof(double2)) : 0)
+             ? 2 * N * TB * int(sizeof(cpx<T>))
+             : TB * (N / 2 + 1) * int(STG == 4 ? sizeof(double4) : sizeof(double2));
+}
+
+// The stage combine of column group `tile` of one solve, with the
+// workgroup's LDS passed in (smraw: zs_lds_bytes, red: (TB * TF) / 64
+// doubles), so that a launch can run the groups of solves of several sizes.
+template <typename T, int N, int SCH, int STG, int TB, int V, bool PROBE>
+__device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsigned char* smraw, double* red) {
   static_assert(!PROBE || (SCH == 4 && STG == 4), "the probe form is the last ETDRK4 stage");
-  const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, V>::E, TF = N / E;
   // ETDRK4 stages 1, 3 and 4 read their coefficient records from a
   // per-workgroup table in LDS: TB columns x (N/2 + 1) folded kz, gathered once
@@ -885,19 +912,16 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   constexpr bool STAGED = SCH == 4 && STG != 2;
   using Rec = typename std::conditional<STG == 4, double4, double2>::type;
   constexpr int NQ = N / 2 + 1;
-  constexpr int FFTB = int(2 * N * TB * sizeof(cpx<T>)), TBLB = STAGED ? int(TB * NQ * sizeof(Rec)) : 0;
-  __shared__ __align__(16) unsigned char smraw[FFTB > TBLB ? FFTB : TBLB];
   cpx<T>* sm = reinterpret_cast<cpx<T>*>(smraw);
-  __shared__ double red[(TB * TF) / 64];
   const int j = threadIdx.x / TB;
   ColLds<T, TB> lds{sm + threadIdx.x % TB};
   ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const Geo g = a.g;
-  const ZIdx id = zidx<TB>(g);
+  const ZIdx id = zidx<TB>(g, tile);
   // The state and U, U2, pre, Na, Nb, Vh are kept group-major: this
   // workgroup's TB columns of all N planes are one contiguous run (see
   // zs_private); the strided z traffic of a stage is only P, Q
-  const long long ib0 = (long long)blockIdx.x * N * TB + threadIdx.x % TB;
+  const long long ib0 = (long long)tile * N * TB + threadIdx.x % TB;
   // (lanes past the last column of a Nyquist group have no input written;
   // they transform zeros and store nothing)
   cpx<T> x[E], x2[E];
@@ -942,7 +966,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
     Rec* w = reinterpret_cast<Rec*>(smraw);
     for (int i = threadIdx.x; i < TB * NQ; i += TB * TF) {
       const int c = i / NQ, q = i % NQ;
-      const ZIdx ic = zidx<TB>(g, blockIdx.x, c);
+      const ZIdx ic = zidx<TB>(g, tile, c);
       const int m = mindex(g, ic.kx, ic.ky, q);
       if constexpr (STG == 4) w[i] = a.c2[m]; else w[i] = a.c1[m];
     }
@@ -1043,7 +1067,16 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, co
   }
   if (id.valid && id.q == 0 && j == 0) a.dc[0] = dc0;
   if constexpr (STAGED) __syncthreads();   // the table's last reads before the transforms reuse LDS
-  next_field_tail<T, N, TB, V, PROBE>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red);
+  next_field_tail<T, N, TB, V, PROBE>(x, y, acc, j, id, g, lds, lds2, a.tw, a.P, a.Q, a.part, red, tile);
+}
+
+template <typename T, int N, int SCH, int STG, int TB = tbw<T, N>(), bool TAB = false, int V = zsplan<T, N>(),
+          bool PROBE = false>
+__global__ void __launch_bounds__((TB * TFof<N, V>())) k_zstage(ZsArgs<T> a0, const ZsArgs<T>* tab) {
+  const ZsArgs<T> a = TAB ? tab[blockIdx.z] : a0;
+  __shared__ __align__(16) unsigned char smraw[zs_lds_bytes<T, N, SCH, STG, TB, V>()];
+  __shared__ double red[(TB * TFof<N, V>()) / 64];
+  zstage_body<T, N, SCH, STG, TB, V, PROBE>(a, int(blockIdx.x), smraw, red);
 }
 
 // Copy of a half spectrum between layouts, widened to FP64: tb_in / tb_out =

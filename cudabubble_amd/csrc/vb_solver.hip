@@ -267,8 +267,11 @@ struct Solver final : SolverBase {
     // padded, 128-byte aligned spectrum rows when every column tile width in
     // use divides nx/2 (see Geo)
     bool tiles = true;
-    for (int t : {ox->ytile, oz->ytile, oxd->ytile, ozd->ytile, ox->ctile, oz->ctile, oxd->ctile, ozd->ctile})
+    for (int t : {ox->ytile, oz->ytile, oxd->ytile, ozd->ytile, ox->ctile, oz->ctile, oxd->ctile, ozd->ctile}) {
+      // the group-major index (gm_index) divides by a column tile with shifts
+      if (t <= 0 || (t & (t - 1)) != 0) throw HipError{hipErrorInvalidValue, "column tile not a power of two"};
       tiles = tiles && (p.nx / 2) % t == 0;
+    }
     if ((p.nx / 2) % 16 == 0 && tiles) {
       g.pitch = p.nx / 2 + 16;
       g.aligned = 1;
@@ -803,6 +806,8 @@ struct Solver final : SolverBase {
   // ---- batched launches (see SolverBase) -------------------------------------
   PipeState rec_pre{}, rec_post{};
   int rec_evals = 0;
+
+  int z_tile() const override { return oz->ytile; }
 
   GroupKey group_key() const override {
     GroupKey k{};

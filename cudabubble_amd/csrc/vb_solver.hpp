@@ -62,6 +62,8 @@ struct SolverBase {
   // valid now; advance_recorded() moves the host state as the two iterations
   // would.
   virtual GroupKey group_key() const = 0;
+  // the column tile of this solve's z kernels (its group-major layout)
+  virtual int z_tile() const = 0;
   virtual int record_pair(Recorder& rec) = 0;
   virtual bool at_recorded_state() const = 0;
   virtual void advance_recorded(vb_step_report* rep) = 0;

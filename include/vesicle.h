@@ -222,6 +222,14 @@ int vb_batch_step(vb_batch* b, int n_steps);
 int vb_batch_report(vb_batch* b, int i, vb_step_report* rep);
 int vb_batch_query(vb_batch* b, int i, double* E, double* V, double* A, double* Eb);
 int vb_batch_get_phi(vb_batch* b, int i, double* host_out);
+/* Shared stage launches across grid sizes (no reference counterpart): with
+ * on != 0, the fixed-dt FP32-mixed ETDRK4 size groups of lengths 32, 64, 128
+ * and 256 (default kernel schedule) run each ETD stage combine as one launch
+ * over the solves of all of them, their other kernels staying one launch per
+ * group; *members (may be NULL) receives the number of size groups that share
+ * (0: fewer than two qualify, and nothing changes).  Results are unchanged bit
+ * for bit.  Call between vb_batch_step calls. */
+int vb_batch_set_shared_stages(vb_batch* b, int on, int* members);
 void vb_batch_destroy(vb_batch* b);
 
 /* ---- request loop ----------------------------------------------------------

@@ -180,3 +180,35 @@ def test_batched_mix_48_solves(vb, gpu):
         assert np.array_equal(s.get_phi(), got[n][0]), n
         assert s.query() == got[n][1], n
         s.close()
+
+
+def test_shared_stage_launches_match_solo(vb, gpu):
+    # solves of four lengths whose ETD stage combines go out as one launch per
+    # stage over all of them (vb_batch_set_shared_stages), beside a size that
+    # a shared launch does not carry (48 x 32): every solve computes exactly
+    # what it computes alone, through the first steps (per-solve launches until
+    # the pipelines recur) and the captured shared graph
+    specs = [dict(nx=32, kind=1, seed=100, M1=1e4, M2=3e4), dict(nx=32, kind=3, seed=101, M1=3e3, M2=1e4),
+             dict(nx=64, kind=4, seed=102, M1=1e4, M2=1e4), dict(nx=64, kind=1, seed=103, M1=2e4, M2=5e3),
+             dict(nx=128, kind=2, seed=104, M1=1e4, M2=1e4), dict(nx=256, kind=3, seed=105, M1=1e4, M2=1e4),
+             dict(nx=48, nz=32, kind=1, seed=106, M1=1e4, M2=1e4)]
+    ps, phis = [], []
+    for sp in specs:
+        sp = dict(sp)
+        p, phi = case(vb, sp.pop("nx"), sp.pop("nz", None), sp.pop("kind"), sp.pop("seed"), precision=1,
+                      phi_lut_mode=1, time_step=1e-7, fixed_time_step=1e-7, **sp)
+        ps.append(p)
+        phis.append(phi)
+    flush_print("shared stage launches: batch of", len(ps))
+    b = vb.Batch(ps, phis)
+    assert b.set_shared_stages(True) == 4          # the 32, 64, 128 and 256 groups
+    reps = b.step(5) + b.step(4)
+    got = [(b.get_phi(i), b.query(i), b.report(i)) for i in range(len(ps))]
+    b.close()
+    for i, (p, phi) in enumerate(zip(ps, phis)):
+        flush_print("shared stage launches: solo run", i)
+        assert reps[i]["status"] == 0 and reps[i]["steps"] == 5, i
+        ref_phi, ref_state, ref_q, _ = solo(vb, p, phi, [5, 4], graphs=0)
+        assert np.array_equal(got[i][0], ref_phi), i
+        assert got[i][1] == ref_q, i
+        assert got[i][2]["time_current"] == ref_state["time_current"], i
