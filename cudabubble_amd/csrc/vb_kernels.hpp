@@ -273,6 +273,34 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
+// K sums over a one-wave workgroup in a fixed order, for K <= 8: every lane
+// stores its K values to LDS (sm: K * 64 doubles), lanes 8q..8q+7 each add
+// eight stored values of sum q, and three exchanges within those eight lanes
+// finish it.  Sums 0-2 go to o3[0..2], sums 3.. to o2[0..] (skipped when o2
+// is null).  Fewer exchanges than one butterfly per sum (block_sum_store: six
+// per sum, each two LDS permutes and an FP64 add).
+template <int K>
+__device__ __forceinline__ void wave_sum_store(const double (&v)[K], double* sm, double* o3, double* o2) {
+  static_assert(K <= 8, "eight lanes per sum");
+  const int l = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < K; ++q) sm[q * 64 + l] = v[q];
+  __syncthreads();
+  const int q = l >> 3, r = l & 7;
+  double s = 0.0;
+  if (q < K) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += sm[q * 64 + r * 8 + i];
+  }
+  s += __shfl_xor(s, 4, 64);
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 1, 64);
+  if (q < K && r == 0) {
+    if (q < 3) o3[q] = s;
+    else if (o2) o2[q - 3] = s;
+  }
+}
+
 // Plan of k_xrow: at 256 the 16 x 16 plan (one LDS exchange per transform
 // instead of two) ran 71 -> 65 us in FP32 (then 204 registers, two waves per
 // SIMD, against 116 and four for 8 x 8 x 4) and raised the FP64 rate 3 % (250
@@ -366,6 +394,17 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   }
   if constexpr (SUMS) {
     block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
+    return;
+  }
+  if constexpr (DCS && NT == 64) {
+    // one wave: the five sums in one exchange (wave_sum_store) after the transform
+    fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
+    unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
+    __shared__ double wred[5 * 64];
+    const double v5[5] = {acc[0], acc[1], acc[2], dcs[0], dcs[1]};
+    double* const o3 = a.part + 3 * (long long)blockIdx.x;
+    double* const o2 = a.dcpart ? a.dcpart + 2 * (long long)blockIdx.x : nullptr;
+    wave_sum_store<5>(v5, wred, o3, o2);
     return;
   }
   if constexpr (DCS) {
