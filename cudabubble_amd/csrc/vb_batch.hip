@@ -55,7 +55,8 @@ struct Shared {
   std::vector<Group*> members;
   hipGraphExec_t gx = nullptr;
   std::vector<std::unique_ptr<vb::DBuf<unsigned char>>> tabs;
-  std::vector<hipEvent_t> events;
+  std::vector<hipEvent_t> events;           // the captured joins and forks
+  std::map<hipStream_t, hipEvent_t> order;  // per member stream: ordering around each graph launch
   void clear() {
     if (gx) (void)hipGraphExecDestroy(gx);
     gx = nullptr;
@@ -63,7 +64,10 @@ struct Shared {
     for (auto e : events) (void)hipEventDestroy(e);
     events.clear();
   }
-  ~Shared() { clear(); }
+  ~Shared() {
+    clear();
+    for (auto& kv : order) (void)hipEventDestroy(kv.second);
+  }
 };
 }  // namespace
 
@@ -154,6 +158,21 @@ struct vb_batch {
     for (auto& kv : by_drift)
       if (kv.second.size() >= 2 && (best < 0 || pts[kv.first] > pts[best])) best = kv.first;
     if (best >= 0) shared.members = by_drift[best];
+  }
+
+  // the distinct streams of the shared members, each with its ordering event
+  std::vector<hipStream_t> member_streams() {
+    std::vector<hipStream_t> v;
+    for (auto* g : shared.members) {
+      hipStream_t s = streams[(size_t)g->stream];
+      if (std::find(v.begin(), v.end(), s) == v.end()) v.push_back(s);
+      if (!shared.order.count(s)) {
+        hipEvent_t e = nullptr;
+        VB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        shared.order[s] = e;
+      }
+    }
+    return v;
   }
 
   bool is_member(const Group* g) const {
@@ -298,7 +317,20 @@ struct vb_batch {
         rec = rec && solves[(size_t)i]->at_recorded_state();
       }
     if (ok && c >= 2 && (rec || build_shared())) {
-      VB_HIP(hipGraphLaunch(shared.gx, streams[(size_t)shared.members[0]->stream]));
+      // The graph is launched on the first member's stream, but the members'
+      // own launches (the per-solve steps before the pipelines recur, an odd
+      // last step) run on their groups' streams: the graph waits for every
+      // member stream, and every member stream waits for the graph.
+      hipStream_t main = streams[(size_t)shared.members[0]->stream];
+      for (hipStream_t s : member_streams())
+        if (s != main) {
+          VB_HIP(hipEventRecord(shared.order[s], s));
+          VB_HIP(hipStreamWaitEvent(main, shared.order[s], 0));
+        }
+      VB_HIP(hipGraphLaunch(shared.gx, main));
+      VB_HIP(hipEventRecord(shared.order[main], main));
+      for (hipStream_t s : member_streams())
+        if (s != main) VB_HIP(hipStreamWaitEvent(s, shared.order[main], 0));
       for (auto* g : shared.members)
         for (int i : g->lanes) solves[(size_t)i]->advance_recorded(&reports[(size_t)i]);
       return 2;

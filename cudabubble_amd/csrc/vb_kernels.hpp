@@ -83,8 +83,10 @@ struct Rows {
 };
 
 // After a forward transform of Z = a + i b (a, b real rows), write the half
-// spectra of a and b, times s.  Uses the lane's LDS; ends with a barrier.
-template <typename T, int N, int V = 0, class Lds>
+// spectra of a and b, times s (and b's times sb as well); SCALE = false writes
+// them unscaled and doubled, Z_k + conj Z_{N-k} and -i (Z_k - conj Z_{N-k}).
+// Uses the lane's LDS; ends with a barrier.
+template <typename T, int N, int V = 0, bool SCALE = true, class Lds>
 __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, cpx<T>* __restrict__ outA,
                                              cpx<T>* __restrict__ outB, T s, bool valid, T sb = T(1)) {
   constexpr int E = Plan<N, V>::E, TF = N / E, NH = N / 2 + 1;
@@ -99,8 +101,13 @@ __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, c
       const cpx<T> zm = lds(k == 0 ? 0 : N - k);
       const T h = T(0.5) * s, hb = h * sb;
       // a_k = (Z_k + conj Z_{N-k}) / 2,  b_k = (Z_k - conj Z_{N-k}) / (2i)
-      outA[k] = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
-      outB[k] = mk<T>((zk.y + zm.y) * hb, (zm.x - zk.x) * hb);
+      if constexpr (SCALE) {
+        outA[k] = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
+        outB[k] = mk<T>((zk.y + zm.y) * hb, (zm.x - zk.x) * hb);
+      } else {
+        outA[k] = mk<T>(zk.x + zm.x, zk.y - zm.y);
+        outB[k] = mk<T>(zk.y + zm.y, zm.x - zk.x);
+      }
     }
   }
   __syncthreads();
@@ -188,9 +195,9 @@ struct XaArgs {
   const cpx<T>* Q;   // x-half-spectra of lap v
   T* v;
   T* f;
-  cpx<T>* Tt;        // out: x-forward of tmp1, times inv_n
+  cpx<T>* Tt;        // out: x-forward of tmp1, times inv_n (k_xrow: doubled, unnormalised)
   cpx<T>* W;         // out (Euler only): x-forward of f, times inv_n
-  cpx<T>* Ts;        // out (k_xrow): x-forward of s, times inv_n
+  cpx<T>* Ts;        // out (k_xrow): x-forward of eps^3 s, doubled, unnormalised
   double* part;      // [gridDim.x][3]
   double* dcpart;    // k_xrow, FP32: [gridDim.x][2] FP64 sums of tmp1 and eps^3 s, or nullptr
   const cpx<T>* tw;
@@ -331,8 +338,11 @@ template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() 
 // (F(tmp1) + kappa v^)/eps.  The stage combine forms N^ from the spectra of
 // tmp1 and s, so lap(tmp1) never goes back to real space.  s is O(1/eps^3)
 // where tmp1 is O(1), and the rounding of a shared transform scales with the
-// larger part: s goes in as eps^3 s, of tmp1's size, and its spectrum is
-// scaled back at the store.
+// larger part: s goes in as eps^3 s, of tmp1's size.  The two spectra are
+// stored unnormalised and doubled (unpack_store without SCALE): 2 x the row
+// transforms of tmp1 and eps^3 s.  The stage combine, which multiplies them by
+// per-mode factors anyway, takes the normalisation 1/(2 N) and the 1/eps^3 into
+// those factors (zstage_body), which saves this kernel four multiplies per mode.
 // SUMS: the partial sums only (no forward transform, nothing stored): the
 // integrals V, A, E_b of a field, as in FP32-mixed mode's FP64 energy of the
 // state (T = double).
@@ -399,7 +409,7 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   if constexpr (DCS && NT == 64) {
     // one wave: the five sums in one exchange (wave_sum_store) after the transform
     fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
-    unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
+    unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(1), valid);
     __shared__ double wred[5 * 64];
     const double v5[5] = {acc[0], acc[1], acc[2], dcs[0], dcs[1]};
     double* const o3 = a.part + 3 * (long long)blockIdx.x;
@@ -414,7 +424,7 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
     }
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
-  unpack_store<T, N, XP>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(a.inv_n), valid, T(1.0 / e3));
+  unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(1), valid);
   block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
@@ -976,26 +986,31 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
   // spectra of tmp1 and s are dead before the combine's operands load:
   // N^ = Np + Av v^ with Av = -lambda2 3/(2 sqrt 2) (eps |k|^2 + kappa/eps).
   // Np stays FP64: near an equilibrium f^ is small and its two parts cancel.
-  fft2<T, N, -1, V>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and s
+  fft2<T, N, -1, V>(x, x2, j, a.tw, lds, lds2);   // spectra of tmp1 and eps^3 s (stored units)
   const double l1h = 0.5 * a.sc->l1;
   const double l2c = a.sc->l2 * (3.0 / (2.0 * sqrt(2.0)));
   const double ie = 1.0 / a.eps;
+  // the x kernel stores 2 N F(tmp1) and 2 N F(eps^3 s) (F normalised): hu and
+  // hs turn them into F(tmp1) and F(s)
+  const double hu = 0.5 / double(g.nr), hs = hu * (ie * ie * ie);
+  const double ieu = ie * hu, etas = a.eta * hs;
   cpx<double> np[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int kz = j + TF * e;
     const double k2 = double(mindex(g, id.kx, id.ky, kz)) * g.kfac2;
-    const double at = -(l2c + a.eta * k2) * ie;   // on F(tmp1)
+    const double at = -(l2c + a.eta * k2) * ieu;   // on the stored spectrum of tmp1
     double tr = double(x[e].x), sr = double(x2[e].x);
     if constexpr (sizeof(T) == 4) {
-      // the k = 0 coefficients summed in FP64 by the x kernel (k_xrow)
+      // the k = 0 coefficients summed in FP64 by the x kernel (k_xrow), the
+      // means of tmp1 and eps^3 s, in stored units
       if (e == 0 && id.q == 0 && j == 0) {
-        tr = a.sc->dc_t1;
-        sr = a.sc->dc_s * (ie * ie * ie);
+        tr = a.sc->dc_t1 / hu;
+        sr = a.sc->dc_s / hu;
       }
     }
-    double pr = at * tr - a.eta * sr;
-    const double pi = at * double(x[e].y) - a.eta * double(x2[e].y);
+    double pr = at * tr - etas * sr;
+    const double pi = at * double(x[e].y) - etas * double(x2[e].y);
     if (id.q == 0 && kz == 0) pr -= l1h;
     np[e] = mk<double>(pr, pi);
   }

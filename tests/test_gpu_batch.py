@@ -182,6 +182,43 @@ def test_batched_mix_48_solves(vb, gpu):
         s.close()
 
 
+def test_batched_mix_48_solves_shared_stages(vb, gpu):
+    # the same mix with the stage combines of the 32^3, 64^3 and 128^3 groups as
+    # one launch each (default kernel schedules), 60 steps in uneven chunks:
+    # every solve ends without a driver exit, and the first solve of every
+    # sharing size matches its solo run bit for bit
+    from cudabubble_amd.workloads import batch_mix
+    ps, phis = batch_mix(vb)
+    saved = {n: vb.schedule_get(n, vb.FP32_MIXED) for n in (32, 64, 128)}
+    try:
+        for n in saved:
+            vb.schedule_set(n, vb.FP32_MIXED, 0)
+        flush_print("48-solve mix, shared stage launches: building")
+        b = vb.Batch(ps, phis)
+        assert b.set_shared_stages(True) == 3
+        chunks = [7, 30, 23]
+        reps = [b.step(c) for c in chunks][-1]
+        statuses = [b.report(i)["status"] for i in range(len(ps))]
+        assert all(st == 0 for st in statuses), [(i, st) for i, st in enumerate(statuses) if st != 0][:6]
+        first = {}
+        for i, p in enumerate(ps):
+            if p.nx in saved:
+                first.setdefault(p.nx, i)
+        got = {n: (b.get_phi(i), b.query(i)) for n, i in first.items()}
+        b.close()
+        for n, i in sorted(first.items()):
+            flush_print("48-solve mix, shared stage launches: solo run of solve", i, "n =", n)
+            s = vb.Solver(ps[i], phis[i])
+            for c in chunks:
+                assert s.step(c)["status"] == 0
+            assert np.array_equal(s.get_phi(), got[n][0]), n
+            assert s.query() == got[n][1], n
+            s.close()
+    finally:
+        for n, sc in saved.items():
+            vb.schedule_set(n, vb.FP32_MIXED, sc)
+
+
 def test_shared_stage_launches_match_solo(vb, gpu):
     # solves of four lengths whose ETD stage combines go out as one launch per
     # stage over all of them (vb_batch_set_shared_stages), beside a size that
@@ -202,13 +239,15 @@ def test_shared_stage_launches_match_solo(vb, gpu):
     flush_print("shared stage launches: batch of", len(ps))
     b = vb.Batch(ps, phis)
     assert b.set_shared_stages(True) == 4          # the 32, 64, 128 and 256 groups
-    reps = b.step(5) + b.step(4)
+    # odd chunks: per-solve steps on the groups' streams before and after the
+    # shared graph's launches on the first member's stream
+    reps = b.step(5) + b.step(3) + b.step(1)
     got = [(b.get_phi(i), b.query(i), b.report(i)) for i in range(len(ps))]
     b.close()
     for i, (p, phi) in enumerate(zip(ps, phis)):
         flush_print("shared stage launches: solo run", i)
         assert reps[i]["status"] == 0 and reps[i]["steps"] == 5, i
-        ref_phi, ref_state, ref_q, _ = solo(vb, p, phi, [5, 4], graphs=0)
+        ref_phi, ref_state, ref_q, _ = solo(vb, p, phi, [5, 3, 1], graphs=0)
         assert np.array_equal(got[i][0], ref_phi), i
         assert got[i][1] == ref_q, i
         assert got[i][2]["time_current"] == ref_state["time_current"], i
