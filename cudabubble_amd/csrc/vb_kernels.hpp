@@ -387,7 +387,8 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
     const T w = v * v - T(1);
     const T f = -lv * T(eps) + w * v * T(ieps);                         // f_kernel, Calc.cu:160
     const T t1 = v * (w - T(kappa));                                    // nld_kernel1, Calc.cu:270
-    const T sv = T(D) * t1 + (T(3) * v * v - T(D + 1)) * (T(eps) * f);  // eps^3 s, Calc.cu:282
+    // eps^3 s = D tmp1 + (3 v^2 - D - 1) eps f (Calc.cu:282), as D tmp1 + (3 eps v^2 - eps (D + 1)) f
+    const T sv = T(D) * t1 + (T(3 * eps) * v * v - T(eps * (D + 1))) * f;
     if constexpr (sizeof(T) == 4) {
       ts[0] += v; ts[1] += w * w; ts[2] += f * f;
       if constexpr (DCS) { ts[3] += t1; ts[4] += sv; }
