@@ -1032,8 +1032,11 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
   // The last stage loads and combines its operands in two halves in FP64, and
   // in FP32 with 12 elements per thread (192, 384): all at once they need 256
   // registers and more in AGPRs, one wave per SIMD; in halves two (4 x 192^3
-  // batched 4.15 -> 3.98 ms per step)
-  constexpr int NH = ((sizeof(T) == 8 || E >= 12) && STG == SCH) ? 2 : 1, EH = E / NH;
+  // batched 4.15 -> 3.98 ms per step).  The probe form does too in FP32 at
+  // 256 (198 -> 166 registers, two waves per SIMD -> three, 121 -> 118 us);
+  // the full FP32 stage 4 stays at two waves either way and ran slower split
+  // (149 -> 154 us).
+  constexpr int NH = ((sizeof(T) == 8 || E >= 12 || PROBE) && STG == SCH) ? 2 : 1, EH = E / NH;
   cpx<T> y[E];
   double acc = 0.0, dc0 = 0.0;
 #pragma unroll
