@@ -43,7 +43,7 @@ class Params(C.Structure):
                 ("fixed_time_step", C.c_double), ("dt_cap", C.c_double),
                 ("dt_double_every", C.c_int), ("penalty_drift", C.c_int),
                 ("phi_lut_mode", C.c_int), ("shape_adjust", C.c_int), ("tmax", C.c_double),
-                ("device", C.c_int), ("graphs", C.c_int)]
+                ("device", C.c_int), ("graphs", C.c_int), ("accept_chain", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

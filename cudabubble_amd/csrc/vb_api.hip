@@ -57,6 +57,8 @@ int vb::validate_params(const vb_params* p) {
   if (p->precision != VB_FP64 && p->precision != VB_FP32_MIXED) return VB_ERR_PARAM;
   if (p->phi_lut_mode != VB_LUT_REFERENCE && p->phi_lut_mode != VB_LUT_ACCURATE) return VB_ERR_PARAM;
   if (!(p->time_step > 0) || p->dt_double_every <= 0 || !(p->dt_cap > 0)) return VB_ERR_PARAM;
+  if (p->accept_chain < 0 || p->accept_chain > 16) return VB_ERR_PARAM;
+  if (p->graphs < 0 || p->graphs > 2) return VB_ERR_PARAM;
   return VB_OK;
 }
 
@@ -154,6 +156,7 @@ void vb_params_default(vb_params* p) {
   p->tmax = 4.0;
   p->device = 0;
   p->graphs = 1;
+  p->accept_chain = 0;
 }
 
 int vb_params_from_ini(const char* path, vb_params* out) {

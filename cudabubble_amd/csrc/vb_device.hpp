@@ -45,6 +45,43 @@ struct Scal {
   int nonfinite, pad;
 };
 
+// The ETD tables of one dt as the stage kernels read them.
+struct LutSel {
+  const double2* c1;
+  const double4* c2;
+};
+
+// Control block of a chain of adaptive ETD attempts whose accept/reject
+// decisions are taken on the device (belastic.cpp:639-669, 713-746).  The host
+// writes it before the chain's launches and reads it back after them; between
+// the two, k_chain_begin / k_chain_end (one per attempt) carry the driver's
+// state: dt, time, the accepted steps, the open iteration's E0.  A chain stops
+// making attempts at the first driver exit, after `budget` accepted
+// iterations, or when dt leaves the tables it was given (CHAIN_NEED_LUT: the
+// host adds the table and goes on).
+constexpr int kChainLevels = 24;
+enum { CHAIN_RUN = 0, CHAIN_FINISHED = 1, CHAIN_NONFINITE = 2, CHAIN_UNDERFLOW = 3, CHAIN_NEED_LUT = 4, CHAIN_BUDGET = 5 };
+struct ChainCtl {
+  double dt, dt_cap;      // the current step; the adaptive cap
+  double t;               // simulated time
+  double E_cur;           // E of the state
+  double E0;              // E of the state when the open iteration began
+  double energy;          // E of the last accepted candidate (NaN until one is)
+  int steps;              // accepted iterations since the start of the run
+  int dt_double_every;
+  int iter_open;          // dt of the current iteration chosen
+  int budget;             // accepted iterations this chain may make
+  int status;             // CHAIN_*
+  int mixed;              // the energy test reads the FP64 scalars
+  int acc, evals, rej;    // this chain's accepted iterations, attempts, rejections
+  int active, accept;     // the current attempt runs; it was accepted
+  int nlev;
+  double lev_dt[kChainLevels];
+  LutSel lev[kChainLevels];
+  LutSel sel;             // tables of the current attempt
+  Scal saved;             // integrals of the state, put back after a rejection
+};
+
 // ETD coefficient record, indexed by the integer m = kx^2 + ky'^2 + kz'^2
 // (folded indices).  ETDRK4: {e1, (e1-1)/H, e2, A s, 2 B s, C s, -, H} with
 // s = 1/(h^2 H^3).  ETDRK2: {e, (e-1)/H, c2, -, -, -, -, H}.

@@ -84,4 +84,20 @@ void launch_finalize(const double* xpart, int nxb, const double* zpart, const do
                      const Geo& g, const Phys& ph, int mode, hipStream_t s);
 void launch_fill_scal(Scal* sc, double alpha, double beta, double alpha0, double beta0, hipStream_t s);
 
+// device-side accept of adaptive attempts (vb_chain.hip): the control kernels
+// before and after an attempt, and the copies of an accepted candidate
+struct ChainCopy {
+  struct Job {
+    const void* src;
+    void* dst;
+    size_t bytes;   // a multiple of 8
+  } job[3];
+  int n;
+};
+void launch_chain_begin(ChainCtl* c, const Scal* sc, hipStream_t s);
+void launch_chain_end(ChainCtl* c, Scal* sc, const Scal* test, hipStream_t s);
+void launch_chain_copy(const ChainCtl* c, const ChainCopy& cp, hipStream_t s);
+// *d = v on stream s (a scalar a captured graph reads)
+void launch_set_double(double* d, double v, hipStream_t s);
+
 }  // namespace vb

@@ -224,12 +224,14 @@ struct Solver final : SolverBase {
   std::vector<hipEvent_t> evpool;
   std::map<std::string, std::pair<int, double>> ktimes;
 
-  // Driver work captured as HIP graphs: two fixed-dt iterations, or one
-  // adaptive attempt.  A graph records the launches that work issues from one
+  // Driver work captured as HIP graphs: two fixed-dt iterations, one
+  // adaptive attempt, a chain of attempts decided on the device, or one Euler
+  // attempt.  A graph records the launches that work issues from one
   // host-side pipeline state (the state index cur, the prepared/prefixed
   // flags and dt) and is replayed only from that same state; afterwards the
   // host state is set to what the work leaves behind, exactly as if it had
-  // run eagerly.
+  // run eagerly.  Chains and Euler attempts read their dt from device memory,
+  // so theirs are replayed whatever dt is.
   struct PipeState {
     int cur;
     bool prepped, prefixed, prefixed_f;
@@ -239,7 +241,7 @@ struct Solver final : SolverBase {
              dt == o.dt;
     }
   };
-  enum { kFixedPair = 0, kAttempt = 1 };
+  enum { kFixedPair = 0, kAttempt = 1, kChain = 2, kShapeAttempt = 3, kEulerAttempt = 4 };
   struct StepGraph {
     hipGraphExec_t x = nullptr;
     int kind = kFixedPair;
@@ -353,6 +355,7 @@ struct Solver final : SolverBase {
     if (att_ev) (void)hipEventDestroy(att_ev);
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (lut_stage) (void)hipHostFree(lut_stage);
+    if (hchain) (void)hipHostFree(hchain);
     flush_prof();
     for (auto e : evpool) (void)hipEventDestroy(e);
     if (hs) (void)hipHostFree(hs);
@@ -586,7 +589,7 @@ struct Solver final : SolverBase {
   // The combine of stage stg: z-forward of tmp1 and s, N^, the ETD combine,
   // and the z-inverse of the next stage field into P, Q.  probe (last ETDRK4
   // stage): the new state is only needed for its V and A, so neither it nor Q
-  // is stored.
+  // is stored.  lt null: the tables of the dt the chain control block chose.
   void suffix(int sch, int stg, const Lut* lt, int out, bool probe = false) {
     ZsArgs<T> z;
     z.U = stg == 1 ? Tt.p : Tt1.p;
@@ -605,8 +608,9 @@ struct Solver final : SolverBase {
     z.Q = Q.p;
     z.part = zpart.p;
     z.dc = dc.p;
-    z.c1 = lt->c1;
-    z.c2 = lt->c2;
+    z.c1 = lt ? lt->c1 : nullptr;
+    z.c2 = lt ? lt->c2 : nullptr;
+    z.lsel = lt ? nullptr : &chain.p->sel;
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
@@ -647,8 +651,9 @@ struct Solver final : SolverBase {
     return out;
   }
 
-  // Forward-Euler update of uh[cur] into uh[1-cur] with rhs g0 (shape) or nld1
-  int euler(double t, bool shape) {
+  // Forward-Euler update of uh[cur] into uh[1-cur] with rhs g0 (shape) or nld1;
+  // tdev: the step t is read from euler_t (set by the caller)
+  int euler(double t, bool shape, bool tdev = false) {
     ensure_prefixed(true);
     const int out = 1 - cur;
     XeArgs<T> e;
@@ -666,6 +671,7 @@ struct Solver final : SolverBase {
     e.shape = shape ? 1 : 0;
     e.rl = Rows{g.ny, g.pitch, g.pp};
     e.part = nullptr;
+    e.tp = tdev ? euler_t.p : nullptr;
     L("xe", [&] { ox->xe(e, st); });
     L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, nullptr, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
@@ -727,7 +733,8 @@ struct Solver final : SolverBase {
       VB_HIP(hipGraphExecDestroy(G.x));
       G.x = nullptr;
     }
-    (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // table upload happens outside the capture
+    if (kind != kShapeAttempt && kind != kEulerAttempt)
+      (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // table upload happens outside the capture
     G.kind = kind;
     G.pre = pipe_state();
     VB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -993,6 +1000,156 @@ struct Solver final : SolverBase {
     return 1;
   }
 
+  // ---- adaptive iterations accepted on the device ----------------------------
+  // With p.accept_chain > 0 the adaptive ETD driver issues chains of up to that
+  // many attempts as one graph launch and reads the driver state back once per
+  // chain: k_chain_begin / k_chain_end (vb_chain.hip) take the decisions of
+  // attempt_prepare() and adaptive_complete() on the device, from the same
+  // doubles.  An attempt of a chain evaluates from uh[cur] into uh[out] with the
+  // tables the control block selects, puts the candidate's prefix into set 1
+  // and, if the candidate is accepted, copies it and that prefix over uh[cur]
+  // and set 0; a rejection leaves both untouched, and its integrals are put
+  // back from the control block.  So the state is always uh[cur], with its
+  // prefix in set 0, and one captured chain serves every dt.
+  // It is not the default: the host's round trip per attempt costs little
+  // beside the attempt's launches.  Measured per attempt (FP32-mixed, 64
+  // iterations from a torus, 20% rejected), host decisions vs chains of 4 and
+  // 16: 118 / 110 / 115 us at 32^3, 181 / 177 / 178 at 64^3, 352 / 352 / 351 at
+  // 128^3, and 1465 / 1501 / 1512 at 256^3, where the accepted candidate's
+  // copies outweigh the round trip.
+  DBuf<ChainCtl> chain;
+  ChainCtl* hchain = nullptr;   // pinned
+  static constexpr int kChainMax = 16;
+
+  bool chain_ok() const {
+    return p.accept_chain > 0 && p.graphs && !prof && p.fixed_time_step <= 0 && p.scheme != VB_EULER;
+  }
+
+  int chain_body(int K) {
+    const int sch = p.scheme == VB_ETDRK2 ? 2 : 4;
+    ChainCopy cp{};
+    cp.n = 3;
+    for (int k = 0; k < K; ++k) {
+      launch_chain_begin(chain.p, scal.p, st);
+      const int keep = cur, out = 1 - cur;
+      for (int stg = 1; stg <= sch; ++stg) {
+        if (stg > 1) prefix(FIN_PLAIN, false, 1);
+        suffix(sch, stg, nullptr, out);
+      }
+      cur = out;
+      prefix(FIN_PLAIN, false, 1);
+      if (kMixed) energy64();
+      cur = keep;
+      launch_chain_end(chain.p, scal.p, kMixed ? scal64.p : scal.p, st);
+      cp.job[0] = {uh[out].p, uh[keep].p, uh[keep].n * sizeof(cpx<double>)};
+      cp.job[1] = {Tt1.p, Tt.p, Tt.n * sizeof(cpx<T>)};
+      cp.job[2] = {Ts1.p, Ts.p, Ts.n * sizeof(cpx<T>)};
+      launch_chain_copy(chain.p, cp, st);
+      check_launch();
+      prepped = false;
+      prefixed = true;
+      prefixed_f = false;
+    }
+    return K;
+  }
+
+  StepGraph* find_chain(int K) {
+    const PipeState now = pipe_state();
+    for (auto& G : graphs)
+      if (G.x && G.kind == kChain && G.evals == K && G.pre.cur == now.cur && G.pre.prepped == now.prepped &&
+          G.pre.prefixed == now.prefixed && G.pre.prefixed_f == now.prefixed_f)
+        return &G;
+    return nullptr;
+  }
+
+  // n iterations (or fewer at a driver exit) of the adaptive ETD driver in
+  // chains; the same report as n rounds of adaptive_issue/adaptive_complete
+  void chain_steps(int n, vb_step_report& r) {
+    const int sch = p.scheme == VB_ETDRK2 ? 2 : 4;
+    if (kMixed) need64();
+    if (!chain.p) {
+      chain.alloc(1);
+      VB_HIP(hipHostMalloc((void**)&hchain, sizeof(ChainCtl), hipHostMallocDefault));
+    }
+    hs_state = false;
+    int done = 0;
+    while (done < n) {
+      if (!iter_open && !cur_E_known) {
+        if (kMixed) {
+          energy64();
+          read_scal64();
+        } else {
+          ensure_prefixed();
+          read_scal();
+        }
+        cur_E = hs->E;
+        cur_E_known = true;
+        hs_state = false;
+      }
+      ensure_prefixed();
+      // tables for the first attempt's dt and its half; the chain asks for
+      // any other it reaches
+      // (every table is in place before the control block lists them: a new
+      // one can take the slot of an old)
+      double d = time_step;
+      if (!iter_open && steps % p.dt_double_every == 0) d = d * 1.4 < p.dt_cap ? d * 2 : p.dt_cap;
+      if (d < 1e-23) d = time_step;
+      (void)lut(sch, time_step);
+      if (d / 2 >= 1e-23) (void)lut(sch, d / 2);
+      const Lut* first = lut(sch, d);
+      const int K = std::min(kChainMax, std::min(p.accept_chain, n - done));
+      ChainCtl& c = *hchain;
+      memset(&c, 0, sizeof(c));
+      c.dt = time_step;
+      c.dt_cap = p.dt_cap;
+      c.t = time_current;
+      c.E_cur = cur_E;
+      c.E0 = att_eng0;
+      c.energy = NAN;
+      c.steps = steps;
+      c.dt_double_every = p.dt_double_every;
+      c.iter_open = iter_open ? 1 : 0;
+      c.budget = K;
+      c.status = CHAIN_RUN;
+      c.mixed = kMixed ? 1 : 0;
+      for (auto& sl : slots)
+        if (sl.sch == sch && c.nlev < kChainLevels) {
+          c.lev_dt[c.nlev] = sl.h;
+          c.lev[c.nlev] = LutSel{sl.c1, sl.c2};
+          c.nlev++;
+        }
+      c.sel = LutSel{first->c1, first->c2};   // read by attempts that do not run
+      VB_HIP(hipMemcpyAsync(chain.p, hchain, sizeof(ChainCtl), hipMemcpyHostToDevice, st));
+      StepGraph* G = find_chain(K);
+      if (!G) G = &capture(kChain, [&] { return chain_body(K); });
+      VB_HIP(hipGraphLaunch(G->x, st));
+      set_pipe_state(G->post);
+      VB_HIP(hipMemcpyAsync(hchain, chain.p, sizeof(ChainCtl), hipMemcpyDeviceToHost, st));
+      VB_HIP(hipStreamSynchronize(st));
+      time_step = c.dt;
+      time_current = c.t;
+      steps = c.steps;
+      iter_open = c.iter_open != 0;
+      att_eng0 = c.E0;
+      cur_E = c.E_cur;
+      cur_E_known = true;
+      r.evaluations += c.evals;
+      r.rejections += c.rej;
+      r.steps += c.acc;
+      done += c.acc;
+      if (!std::isnan(c.energy)) r.energy = c.energy;
+      r.time_step = time_step;
+      r.time_current = time_current;
+      if (c.status == CHAIN_FINISHED) r.status = VB_FINISHED;
+      else if (c.status == CHAIN_NONFINITE) r.status = VB_ERR_NONFINITE;
+      else if (c.status == CHAIN_UNDERFLOW) {
+        r.status = VB_ERR_DT_UNDERFLOW;
+        iter_open = false;   // as adaptive_complete() after an attempt that was not issued
+      }
+      if (r.status != VB_OK) break;
+    }
+  }
+
   void launch_steps(int n, vb_step_report* rep) override {
     hs_state = false;
     if (!started) start();
@@ -1034,6 +1191,40 @@ struct Solver final : SolverBase {
   // ---- drivers ----------------------------------------------------------------
   void set_targets_device() { launch_fill_scal(scal.p, alpha, beta, alpha0, beta0, st); }
 
+  // One Euler attempt of the shape-adjust and run_static loops: the update of
+  // uh[cur] with step t into uh[out], and the prefix of the candidate (its
+  // integrals in scal; v, f and W for the next update if it is kept), which
+  // becomes cur.  With graphs = 2 the launches replay one captured graph per
+  // pipeline state and kind, whatever t: the update reads t from euler_t.
+  // Eager launches are the default: measured per attempt (FP32-mixed, the host
+  // reading E after each), the graph form took 71 / 97 / 153 us at 32^3 /
+  // 64^3 / 128^3 against 80 / 93 / 142 us eager.
+  DBuf<double> euler_t;
+  void euler_attempt(double t, bool shape) {
+    auto body = [&](bool tdev) {
+      const int out = euler(t, shape, tdev);
+      prep(uh[out].p);
+      cur = out;
+      prefix(FIN_PLAIN, true);
+    };
+    if (p.graphs < 2 || prof) return body(false);
+    if (!euler_t.p) euler_t.alloc(1);
+    launch_set_double(euler_t.p, t, st);
+    const int kind = shape ? kShapeAttempt : kEulerAttempt;
+    StepGraph* G = nullptr;
+    const PipeState now = pipe_state();
+    for (auto& H : graphs)
+      if (H.x && H.kind == kind && H.pre.cur == now.cur && H.pre.prepped == now.prepped &&
+          H.pre.prefixed == now.prefixed && H.pre.prefixed_f == now.prefixed_f)
+        G = &H;
+    if (!G) G = &capture(kind, [&] {
+      body(true);
+      return 1;
+    });
+    VB_HIP(hipGraphLaunch(G->x, st));
+    set_pipe_state(G->post);
+  }
+
   // Forward-Euler shape relaxation, unconstrained (belastic.cpp:174-283)
   void shape_adjust() {
     double tcur = 0.0;
@@ -1054,11 +1245,8 @@ struct Solver final : SolverBase {
       }
       if (nst % 10 == 0) dt *= 2;
       for (;;) {
-        const int out = euler(-dt * 3, true);
-        prep(uh[out].p);
         const int keep = cur;
-        cur = out;
-        prefix(FIN_PLAIN, true);
+        euler_attempt(-dt * 3, true);
         read_scal();
         eng1 = hs->Eu;
         if (eng1 < eng0) {
@@ -1117,12 +1305,9 @@ struct Solver final : SolverBase {
           if (steps % 4 == 0) time_step = time_step * 1.4;
           for (;;) {
             if (time_step < 1e-23) break;
-            const int out = euler(-time_step * p.ga, false);
-            r.evaluations++;
-            prep(uh[out].p);
             const int keep = cur;
-            cur = out;
-            prefix(FIN_PLAIN, true);
+            euler_attempt(-time_step * p.ga, false);
+            r.evaluations++;
             read_scal();
             if (hs->E > eng0) {
               cur = keep;
@@ -1144,6 +1329,10 @@ struct Solver final : SolverBase {
         if (time_step < 1e-23) { r.status = VB_ERR_DT_UNDERFLOW; break; }
         if (time_current > p.tmax) { r.status = VB_FINISHED; break; }
       } else if (adaptive) {
+        if (chain_ok()) {   // the decisions on the device, a read-back per chain
+          chain_steps(n - it, r);
+          break;
+        }
         // belastic.cpp:639-669: attempts until one is accepted
         int acc = 0;
         do {
@@ -1341,6 +1530,7 @@ struct Solver final : SolverBase {
     e.shape = 0;
     e.rl = Rows{g.ny, g.pitch, g.pp};
     e.part = rpart.p;
+    e.tp = nullptr;
     L("xe_resid", [&] { ox->xe(e, st); });
     check_launch();
     std::vector<double> h((size_t)nxb);

@@ -65,9 +65,17 @@ typedef struct vb_params {
   int shape_adjust;        /* b_shape: Euler pre-relaxation at t = 0            */
   double tmax;             /* Euler driver stop time (belastic.cpp:407)         */
   int device;              /* HIP device ordinal                                */
-  int graphs;              /* 1: fixed-dt iterations replay captured HIP graphs
-                              (same kernels and arguments as eager launches);
+  int graphs;              /* 1: fixed-dt iterations and adaptive ETD attempts
+                              replay captured HIP graphs (same kernels and
+                              arguments as eager launches); 2: also the Euler
+                              attempts of shape_adjust and of the adaptive
+                              VB_EULER driver, one graph for every dt;
                               0: eager launches only                            */
+  int accept_chain;        /* adaptive ETD with graphs: > 0: up to this many
+                              attempts (at most 16) per launch, accepted or
+                              rejected on the device with the host driver's
+                              arithmetic, one read-back per launch; 0: the host
+                              decides after each attempt (belastic.cpp:639-669) */
 } vb_params;
 
 /* Counters of one vb_step call. */

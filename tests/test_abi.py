@@ -42,6 +42,7 @@ def test_params_default_matches_reference_defaults(vb):
     assert (p.kappa, p.D) == (2.0, 1.5)              # Calc.cu:581-582
     assert p.dt_cap == 1e-3 and p.dt_double_every == 4   # belastic.cpp:644-651
     assert p.penalty_drift == 1 and p.scheme == vb.ETDRK4
+    assert p.accept_chain == 0                       # the host decides each attempt, as the reference
 
 
 @pytest.mark.parametrize("code,mult", [(0, 2.0 * 2), (1, 2.5 * np.sqrt(2.0)), (2, 2.5 * 2), (3, 1.5 * 2),
@@ -231,7 +232,8 @@ def test_autotune_arguments_and_no_device(vb):
 @pytest.mark.parametrize("field,value,status", [
     ("dt_double_every", 0, "VB_ERR_PARAM"), ("time_step", 0.0, "VB_ERR_PARAM"), ("dt_cap", -1.0, "VB_ERR_PARAM"),
     ("scheme", 3, "VB_ERR_PARAM"), ("precision", 2, "VB_ERR_PARAM"), ("phi_lut_mode", 5, "VB_ERR_PARAM"),
-    ("eps", 0.0, "VB_ERR_PARAM"), ("nx", 33, "VB_ERR_ODD_NX"), ("nx", 100, "VB_ERR_PARAM")])
+    ("eps", 0.0, "VB_ERR_PARAM"), ("nx", 33, "VB_ERR_ODD_NX"), ("nx", 100, "VB_ERR_PARAM"),
+    ("accept_chain", -1, "VB_ERR_PARAM"), ("accept_chain", 17, "VB_ERR_PARAM"), ("graphs", 3, "VB_ERR_PARAM")])
 def test_invalid_params_rejected_by_every_entry_point(vb, field, value, status):
     # vb_create, vb_footprint and vb_batch_create share one check, made before
     # any device work (a zero doubling period would otherwise divide by zero in
