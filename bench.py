@@ -324,6 +324,64 @@ def copy_bandwidth(repeats=20):
     return 2 * n * 4 / sec
 
 
+def driver_forms():
+    """Wall time per attempt of the driver forms on one solve: adaptive ETD with
+    host decisions vs chains decided on the device (accept_chain 4 and 16), and
+    Euler attempts eager vs in graph form (graphs = 2); FP32-mixed, one JSON
+    line per grid.  Each pair is also checked to end in the same field."""
+    import time
+    import numpy as np
+    import cudabubble_amd as vb
+
+    def adaptive(n, ac, steps):
+        eps = 2.0 * 2 * np.pi / n
+        phi, _ = vb.shape(3, 20260002, n, n, 1.0, eps)
+        p = vb.params_default(nx=n, nz=n, eps=eps, M1=1e4, M2=1e4, alpha=-1.0, beta=-1.0, time_step=1e-7,
+                              precision=vb.FP32_MIXED, accept_chain=ac)
+        s = vb.Solver(p, phi)
+        s.step(steps)                 # graphs and tables in place
+        t0 = time.perf_counter()
+        r = s.step(steps)
+        sec = time.perf_counter() - t0
+        out = s.get_phi()
+        s.close()
+        return 1e6 * sec / r["evaluations"], r, out
+
+    def euler(n, graphs):
+        eps = 2.0 * 2 * np.pi / n
+        phi, _ = vb.shape(2, 20260002, n, n, 1.0, eps)
+        p = vb.params_default(nx=n, nz=n, eps=eps, scheme=vb.EULER, M1=1e3, M2=1e3, alpha=-1.0, beta=-1.0,
+                              time_step=1e-7, fixed_time_step=0.0, precision=vb.FP32_MIXED, graphs=graphs)
+        s = vb.Solver(p, phi)
+        s.step(20)
+        t0 = time.perf_counter()
+        r = s.step(200)
+        sec = time.perf_counter() - t0
+        out = s.get_phi()
+        s.close()
+        return 1e6 * sec / r["evaluations"], out
+
+    for n in (32, 64, 128, 256):
+        steps = 64 if n <= 128 else 24
+        line = {"grid": n, "precision": "fp32_mixed", "adaptive_iterations": steps}
+        base = None
+        for ac in (0, 4, 16):
+            us, r, out = adaptive(n, ac, steps)
+            line["us_per_attempt_chain%d" % ac] = round(us, 1)
+            if base is None:
+                base = (r, out)
+                line["attempts"], line["rejections"] = r["evaluations"], r["rejections"]
+            else:
+                line["same_chain%d" % ac] = bool(np.array_equal(out, base[1]) and r == base[0])
+        if n <= 128:
+            e0, o0 = euler(n, 0)
+            e2, o2 = euler(n, 2)
+            line["euler_us_per_attempt_eager"] = round(e0, 1)
+            line["euler_us_per_attempt_graph"] = round(e2, 1)
+            line["euler_same"] = bool(np.array_equal(o0, o2))
+        print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,9 +404,15 @@ def main():
                     help="keep schedule 0 instead of autotuning the kernel schedules of each grid size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--driver-forms", action="store_true",
+                    help="run only the per-attempt timing of the driver forms (host vs device "
+                         "decisions, eager vs graph Euler attempts)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.driver_forms:
+        driver_forms()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver path runs on a single device and "

@@ -2,7 +2,8 @@
 //
 // A chain is a run of attempts issued back to back without the host: before
 // each attempt k_chain_begin opens the iteration (E0 and the dt doubling of
-// belastic.cpp:639-651) and picks the coefficient tables of the dt; after it
+// belastic.cpp:639-651) and picks the coefficient tables of the dt, which
+// k_chain_table copies to where the stage kernels read them; after it
 // k_chain_end applies the energy test (belastic.cpp:653-668) and the exits of
 // belastic.cpp:713-746; k_chain_copy then makes an accepted candidate the
 // state.  The arithmetic is the host driver's (the same doubles, compared and
@@ -93,6 +94,18 @@ __global__ void __launch_bounds__(256) k_chain_copy(const ChainCtl* c, ChainCopy
   }
 }
 
+// the tables of the current attempt's dt into the chain's own table buffers,
+// which the stage kernels read (nrec records each)
+__global__ void __launch_bounds__(256) k_chain_table(const ChainCtl* c, double2* c1, double4* c2, int nrec) {
+  if (!c->active) return;
+  const double2* s1 = c->sel.c1;
+  const double4* s2 = c->sel.c2;
+  for (int i = int(blockIdx.x * 256 + threadIdx.x); i < nrec; i += int(gridDim.x * 256)) {
+    c1[i] = s1[i];
+    c2[i] = s2[i];
+  }
+}
+
 __global__ void __launch_bounds__(64) k_set_double(double* d, double v) {
   if (threadIdx.x == 0) *d = v;
 }
@@ -105,6 +118,11 @@ void launch_set_double(double* d, double v, hipStream_t s) {
 
 void launch_chain_begin(ChainCtl* c, const Scal* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_chain_begin, dim3(1), dim3(64), 0, s, c, sc);
+}
+
+void launch_chain_table(const ChainCtl* c, double2* c1, double4* c2, int nrec, hipStream_t s) {
+  const unsigned blocks = unsigned(std::min(1024, std::max(1, (nrec + 255) / 256)));
+  hipLaunchKernelGGL(k_chain_table, dim3(blocks), dim3(256), 0, s, c, c1, c2, nrec);
 }
 
 void launch_chain_end(ChainCtl* c, Scal* sc, const Scal* test, hipStream_t s) {

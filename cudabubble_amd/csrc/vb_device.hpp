@@ -45,7 +45,7 @@ struct Scal {
   int nonfinite, pad;
 };
 
-// The ETD tables of one dt as the stage kernels read them.
+// The ETD tables of one dt.
 struct LutSel {
   const double2* c1;
   const double4* c2;

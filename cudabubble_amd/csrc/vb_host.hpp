@@ -95,6 +95,7 @@ struct ChainCopy {
   int n;
 };
 void launch_chain_begin(ChainCtl* c, const Scal* sc, hipStream_t s);
+void launch_chain_table(const ChainCtl* c, double2* c1, double4* c2, int nrec, hipStream_t s);
 void launch_chain_end(ChainCtl* c, Scal* sc, const Scal* test, hipStream_t s);
 void launch_chain_copy(const ChainCtl* c, const ChainCopy& cp, hipStream_t s);
 // *d = v on stream s (a scalar a captured graph reads)

@@ -910,7 +910,6 @@ struct ZsArgs {
   double* dc;
   const double2* c1;      // per m: ETDRK4 {e1, (e1-1)/H}; ETDRK2 {e^z, (e^z-1)/H}
   const double4* c2;      // per m: ETDRK4 {e2, A s, 2 B s, C s}; ETDRK2 {c2, 0, 0, 0}
-  const LutSel* lsel;     // non-null: c1 and c2 are read from here (the dt chosen on the device)
   const cpx<T>* tw;
   Geo g;
 };
@@ -970,12 +969,6 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
   ColLds<T, TB> lds2{sm + N * TB + threadIdx.x % TB};
   const Geo g = a.g;
   const ZIdx id = zidx<TB>(g, tile);
-  const double2* c1t = a.c1;
-  const double4* c2t = a.c2;
-  if (a.lsel) {
-    c1t = a.lsel->c1;
-    c2t = a.lsel->c2;
-  }
   // The state and U, U2, pre, Na, Nb, Vh are kept group-major: this
   // workgroup's TB columns of all N planes are one contiguous run (see
   // zs_private); the strided z traffic of a stage is only P, Q
@@ -1031,7 +1024,7 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
       const int c = i / NQ, q = i % NQ;
       const ZIdx ic = zidx<TB>(g, tile, c);
       const int m = mindex(g, ic.kx, ic.ky, q);
-      if constexpr (STG == 4) w[i] = c2t[m]; else w[i] = c1t[m];
+      if constexpr (STG == 4) w[i] = a.c2[m]; else w[i] = a.c1[m];
     }
     __syncthreads();
   }
@@ -1082,25 +1075,25 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
       x[e] = mk<T>(T(nr), T(ni));
       double fr, fi;   // next field
       if constexpr (SCH == 4 && STG == 1) {
-        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : c1t[m];   // e1, q
+        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c1[m];   // e1, q
         const cpx<double> u = ur[iq];
         fr = c.x * u.x + c.y * nr;                      // etd41: a^
         fi = c.x * u.y + c.y * ni;
         if (id.valid) st_nt(a.pre + ib, mk<T>(T(c.x * fr - c.y * nr), T(c.x * fi - c.y * ni)));
       } else if constexpr (SCH == 4 && STG == 2) {
-        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : c1t[m];
+        const double2 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c1[m];
         const cpx<double> u = ur[iq];
         fr = c.x * u.x + c.y * nr;                      // etd42: b^
         fi = c.x * u.y + c.y * ni;
         if (id.valid) st_nt(a.Na + ib, x[e]);
       } else if constexpr (SCH == 4 && STG == 3) {
-        const double q = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)].y : c1t[m].y;
+        const double q = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)].y : a.c1[m].y;
         const cpx<T> p = pa[iq];
         fr = double(p.x) + 2.0 * q * nr;                // etd43: c^ = e1 F(a) + q (2 N^b - N^u)
         fi = double(p.y) + 2.0 * q * ni;
         if (id.valid) st_nt(a.Nb + ib, x[e]);
       } else if constexpr (SCH == 4 && STG == 4) {
-        const double4 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : c2t[m];   // e2, A s, 2 B s, C s
+        const double4 c = STAGED ? tbl[tcol * NQ + (kz <= N / 2 ? kz : N - kz)] : a.c2[m];   // e2, A s, 2 B s, C s
         const cpx<double> u = ur[iq];
         const cpx<T> na = pa[iq], nb = pb[iq];
         // etd44: e2 u^ + [N^a A + 2 (N^a + N^b) B + N^c C] s  (first term N^a as in Calc.cu:559)
@@ -1108,14 +1101,14 @@ __device__ __forceinline__ void zstage_body(const ZsArgs<T>& a, int tile, unsign
         fi = c.x * u.y + (double(na.y) * c.y + (double(na.y) + double(nb.y)) * c.z + ni * c.w);
         if (id.valid && !PROBE) st_nt(a.uout + ib, mk<double>(fr, fi));
       } else if constexpr (SCH == 2 && STG == 1) {
-        const double2 c = c1t[m];                      // e^z, q1
+        const double2 c = a.c1[m];                      // e^z, q1
         const cpx<double> u = ur[iq];
         fr = c.x * u.x + c.y * nr;                      // etd1: a^
         fi = c.x * u.y + c.y * ni;
         if (id.valid) st_nt(a.Na + ib, x[e]);
       } else {                                          // SCH == 2 && STG == 2
-        const double2 c = c1t[m];
-        const double c2 = c2t[m].x;
+        const double2 c = a.c1[m];
+        const double c2 = a.c2[m].x;
         const cpx<double> u = ur[iq];
         const cpx<T> nu = pa[iq];
         // etd2: u^+ = F(a) + c2 (N^a - N^u), a^ = e^z u^ + q1 N^u

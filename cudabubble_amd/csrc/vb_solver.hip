@@ -589,7 +589,7 @@ struct Solver final : SolverBase {
   // The combine of stage stg: z-forward of tmp1 and s, N^, the ETD combine,
   // and the z-inverse of the next stage field into P, Q.  probe (last ETDRK4
   // stage): the new state is only needed for its V and A, so neither it nor Q
-  // is stored.  lt null: the tables of the dt the chain control block chose.
+  // is stored.  lt null: the chain's table buffers (the dt its control block chose).
   void suffix(int sch, int stg, const Lut* lt, int out, bool probe = false) {
     ZsArgs<T> z;
     z.U = stg == 1 ? Tt.p : Tt1.p;
@@ -608,9 +608,8 @@ struct Solver final : SolverBase {
     z.Q = Q.p;
     z.part = zpart.p;
     z.dc = dc.p;
-    z.c1 = lt ? lt->c1 : nullptr;
-    z.c2 = lt ? lt->c2 : nullptr;
-    z.lsel = lt ? nullptr : &chain.p->sel;
+    z.c1 = lt ? lt->c1 : chain_c1.p;
+    z.c2 = lt ? lt->c2 : chain_c2.p;
     z.tw = twz.p;
     z.g = g;
     static const char* names[5] = {"zstage", "zstage1", "zstage2", "zstage3", "zstage4"};
@@ -1006,7 +1005,8 @@ struct Solver final : SolverBase {
   // chain: k_chain_begin / k_chain_end (vb_chain.hip) take the decisions of
   // attempt_prepare() and adaptive_complete() on the device, from the same
   // doubles.  An attempt of a chain evaluates from uh[cur] into uh[out] with the
-  // tables the control block selects, puts the candidate's prefix into set 1
+  // tables the control block selects (copied to chain_c1, chain_c2 by
+  // k_chain_table), puts the candidate's prefix into set 1
   // and, if the candidate is accepted, copies it and that prefix over uh[cur]
   // and set 0; a rejection leaves both untouched, and its integrals are put
   // back from the control block.  So the state is always uh[cur], with its
@@ -1019,6 +1019,8 @@ struct Solver final : SolverBase {
   // copies outweigh the round trip.
   DBuf<ChainCtl> chain;
   ChainCtl* hchain = nullptr;   // pinned
+  DBuf<double2> chain_c1;       // the tables of the current attempt's dt
+  DBuf<double4> chain_c2;
   static constexpr int kChainMax = 16;
 
   bool chain_ok() const {
@@ -1031,6 +1033,7 @@ struct Solver final : SolverBase {
     cp.n = 3;
     for (int k = 0; k < K; ++k) {
       launch_chain_begin(chain.p, scal.p, st);
+      launch_chain_table(chain.p, chain_c1.p, chain_c2.p, mmax + 1, st);
       const int keep = cur, out = 1 - cur;
       for (int stg = 1; stg <= sch; ++stg) {
         if (stg > 1) prefix(FIN_PLAIN, false, 1);
@@ -1069,6 +1072,8 @@ struct Solver final : SolverBase {
     if (kMixed) need64();
     if (!chain.p) {
       chain.alloc(1);
+      chain_c1.alloc((size_t)mmax + 1);
+      chain_c2.alloc((size_t)mmax + 1);
       VB_HIP(hipHostMalloc((void**)&hchain, sizeof(ChainCtl), hipHostMallocDefault));
     }
     hs_state = false;
