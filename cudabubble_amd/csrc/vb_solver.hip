@@ -1014,8 +1014,8 @@ struct Solver final : SolverBase {
   // It is not the default: the host's round trip per attempt costs little
   // beside the attempt's launches.  Measured per attempt (FP32-mixed, 64
   // iterations from a torus, 20% rejected), host decisions vs chains of 4 and
-  // 16: 118 / 110 / 115 us at 32^3, 181 / 177 / 178 at 64^3, 352 / 352 / 351 at
-  // 128^3, and 1465 / 1501 / 1512 at 256^3, where the accepted candidate's
+  // 16: 118 / 115 / 120 us at 32^3, 181 / 178 / 182 at 64^3, 353 / 353 / 354 at
+  // 128^3, and 1457 / 1493 / 1507 at 256^3, where the accepted candidate's
   // copies outweigh the round trip.
   DBuf<ChainCtl> chain;
   ChainCtl* hchain = nullptr;   // pinned
@@ -1202,8 +1202,8 @@ struct Solver final : SolverBase {
   // becomes cur.  With graphs = 2 the launches replay one captured graph per
   // pipeline state and kind, whatever t: the update reads t from euler_t.
   // Eager launches are the default: measured per attempt (FP32-mixed, the host
-  // reading E after each), the graph form took 71 / 97 / 153 us at 32^3 /
-  // 64^3 / 128^3 against 80 / 93 / 142 us eager.
+  // reading E after each), the graph form took 72 / 97 / 152 us at 32^3 /
+  // 64^3 / 128^3 against 76 / 91 / 142 us eager.
   DBuf<double> euler_t;
   void euler_attempt(double t, bool shape) {
     auto body = [&](bool tdev) {
