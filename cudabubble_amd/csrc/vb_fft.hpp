@@ -65,6 +65,7 @@ template <> struct Plan<512, 0> { static constexpr int E = 8,  NP = 3; static co
 template <int N> struct Plan<N, 1> : Plan<N, 0> {};
 template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct Plan<128, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 8}; };
+template <> struct Plan<512, 1> { static constexpr int E = 16, NP = 3; static constexpr int R[3] = {16, 8, 4}; };
 
 template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 

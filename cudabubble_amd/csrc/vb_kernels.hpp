@@ -49,11 +49,15 @@ template <typename T, int N> constexpr int colw() {
 }
 
 // Column tile of the z stage kernels (stage combines, z-prep, plain z
-// passes): the column width, except FP32 at 256, where 8 columns (256-thread
-// workgroups, two or three per CU) measured 6-14 % faster per stage than 16:
-// their inputs are group-major, so only their P, Q outputs are written in
-// 64-byte pieces.
-template <typename T, int N> constexpr int tbw() { return (sizeof(T) == 4 && N == 256) ? 8 : colw<T, N>(); }
+// passes): the column width, except FP32 at 256 and 512, 8 columns.  At 256
+// (256-thread workgroups, two or three per CU) 8 measured 6-14 % faster per
+// stage than 16: their inputs are group-major, so only their P, Q outputs are
+// written in 64-byte pieces.  At 512, 16 columns make 1024-thread workgroups,
+// whose 128-register budget spilled 160-412 bytes per lane in the stage
+// combines; 8 took the combines from 1410-2358 to 1118-1440 us (the forward y
+// pass, which writes the 8-column groups, from 483 to 590 us) and a step from
+// 25.7 to 23.2 ms.
+template <typename T, int N> constexpr int tbw() { return (sizeof(T) == 4 && (N == 256 || N == 512)) ? 8 : colw<T, N>(); }
 // The other z tile a kernel schedule can select: the column width where tbw
 // is narrower, else half of it.
 template <typename T, int N> constexpr int tbw_alt() { return tbw<T, N>() < colw<T, N>() ? colw<T, N>() : tbw<T, N>() / 2; }
@@ -312,9 +316,10 @@ __device__ __forceinline__ void wave_sum_store(const double (&v)[K], double* sm,
 // instead of two) ran 71 -> 65 us in FP32 (then 204 registers, two waves per
 // SIMD, against 116 and four for 8 x 8 x 4) and raised the FP64 rate 3 % (250
 // registers, two waves); at 128 in FP32 the 16 x 8 plan (one exchange instead
-// of two).  The FP64 x kernel at 128 keeps 8 x 4 x 4.
+// of two); at 512 in FP32 16 x 8 x 4 (half the threads per row of 8 x 8 x 8,
+// 760 -> 688 us).  The FP64 x kernel at 128 keeps 8 x 4 x 4.
 template <typename T, int N> constexpr int xrplan() {
-  return (N == 256 || (N == 128 && sizeof(T) == 4)) ? 1 : xplan<T, N>();
+  return (N == 256 || ((N == 128 || N == 512) && sizeof(T) == 4)) ? 1 : xplan<T, N>();
 }
 // the other plan a kernel schedule can select (the same where no two-pass
 // plan exists)
