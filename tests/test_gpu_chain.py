@@ -36,22 +36,27 @@ def run(vb, p, phi, calls):
     return out
 
 
-@pytest.mark.parametrize("n,prec,chain,scheme", [
-    (32, "FP64", 4, "ETDRK4"), (64, "FP32_MIXED", 16, "ETDRK4"), (64, "FP64", 3, "ETDRK4"),
-    (32, "FP32_MIXED", 5, "ETDRK2")])
-def test_chain_matches_host_decisions(vb, gpu, n, prec, chain, scheme):
+@pytest.mark.parametrize("n,prec,chain,scheme,cap", [
+    (32, "FP64", 4, "ETDRK4", 1e-3), (64, "FP32_MIXED", 16, "ETDRK4", 1e-3), (64, "FP64", 3, "ETDRK4", 1e-3),
+    (32, "FP32_MIXED", 5, "ETDRK2", 1e-3), (32, "FP32_MIXED", 8, "ETDRK4", 5e-7)])
+def test_chain_matches_host_decisions(vb, gpu, n, prec, chain, scheme, cap):
     # dt starts at 1e-4, far above what the penalties allow, so the first
     # iterations reject several times and halve dt below the tables a chain
-    # starts with; the calls of 1, 7, 13 and 19 steps end chains early
+    # starts with; the calls of 1, 7, 13 and 19 steps end chains early.  With
+    # the 5e-7 cap the doubling rule sets dt = cap instead of 2 dt
+    # (belastic.cpp:644-651) at the first iteration and every fourth after.
     calls = [1, 7, 13, 19]
-    flush_print("chain", n, prec, chain, scheme)
+    flush_print("chain", n, prec, chain, scheme, cap)
     reps = {}
     for ac in (0, chain):
-        p, phi = make(vb, n, M1=1e4, M2=1e4, alpha=-1.0, beta=-1.0, time_step=1e-4,
+        p, phi = make(vb, n, M1=1e4, M2=1e4, alpha=-1.0, beta=-1.0, time_step=1e-4, dt_cap=cap,
                       precision=getattr(vb, prec), scheme=getattr(vb, scheme), accept_chain=ac)
         reps[ac] = run(vb, p, phi, calls)
     (r0, phi0, q0, s0), (r1, phi1, q1, s1) = reps[0], reps[chain]
-    assert sum(r["rejections"] for r in r0) >= 3, r0
+    if cap < 1e-3:
+        assert all(r["time_step"] <= cap for r in r0) and any(r["time_step"] == cap for r in r0), r0
+    else:
+        assert sum(r["rejections"] for r in r0) >= 3, r0
     for a, b in zip(r0, r1):
         assert same_report(a, b), (a, b)
     assert np.array_equal(phi0, phi1)
