@@ -110,8 +110,9 @@ def test_euler_attempt_graphs_match_eager(vb, gpu, prec):
         s.step(1)   # the driver start runs the shape adjustment
         shaped = s.get_phi()
         s.close()
+        # from dt = 1e-5 the explicit update first raises E: rejections halve dt
         p, phi = make(vb, n, kind=2, scheme=vb.EULER, M1=1e3, M2=1e3, alpha=-1.0, beta=-1.0, eta=1.7,
-                      time_step=1e-7, fixed_time_step=0.0, precision=getattr(vb, prec), graphs=graphs)
+                      time_step=1e-5, fixed_time_step=0.0, precision=getattr(vb, prec), graphs=graphs)
         s = vb.Solver(p, phi)
         reps = [s.step(5) for _ in range(4)]
         out[graphs] = (shaped, reps, s.get_phi(), s.query())
@@ -121,4 +122,5 @@ def test_euler_attempt_graphs_match_eager(vb, gpu, prec):
         assert same_report(a, b), (a, b)
     flush_print("Euler attempts", sum(r["evaluations"] for r in out[0][1]),
                 "rejections", sum(r["rejections"] for r in out[0][1]))
+    assert sum(r["rejections"] for r in out[0][1]) > 0   # the rejection path ran in graph form too
     assert np.array_equal(out[0][2], out[2][2]) and out[0][3] == out[2][3]
