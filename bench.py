@@ -361,6 +361,21 @@ def driver_forms():
         s.close()
         return 1e6 * sec / r["evaluations"], out
 
+    def shape(n, graphs, ac):
+        # the shape-adjust pre-pass alone (the driver start of a fixed-dt run)
+        eps = 2.0 * 2 * np.pi / n
+        phi, _ = vb.shape(1, 20260002, n, n, 1.0, eps)
+        p = vb.params_default(nx=n, nz=n, eps=eps, shape_adjust=1, M1=1e4, M2=1e4, alpha=-1.0, beta=-1.0,
+                              time_step=1e-6, fixed_time_step=1e-6, precision=vb.FP32_MIXED, graphs=graphs,
+                              accept_chain=ac)
+        s = vb.Solver(p, phi)
+        t0 = time.perf_counter()
+        s.step(0)
+        sec = time.perf_counter() - t0
+        out = s.get_phi()
+        s.close()
+        return sec, out
+
     for n in (32, 64, 128, 256):
         steps = 64 if n <= 128 else 24
         line = {"grid": n, "precision": "fp32_mixed", "adaptive_iterations": steps}
@@ -379,6 +394,14 @@ def driver_forms():
             line["euler_us_per_attempt_eager"] = round(e0, 1)
             line["euler_us_per_attempt_graph"] = round(e2, 1)
             line["euler_same"] = bool(np.array_equal(o0, o2))
+            shape(n, 1, 8)   # warm: the kernels' first launches in the process (each solve captures its own graphs)
+            se, so = shape(n, 1, 0)
+            sg, sgo = shape(n, 2, 0)
+            sc, sco = shape(n, 1, 8)
+            line["shape_adjust_ms_eager"] = round(1e3 * se, 2)
+            line["shape_adjust_ms_graph"] = round(1e3 * sg, 2)
+            line["shape_adjust_ms_device_decisions"] = round(1e3 * sc, 2)
+            line["shape_same"] = bool(np.array_equal(so, sgo) and np.array_equal(so, sco))
         print(json.dumps(line), flush=True)
 
 

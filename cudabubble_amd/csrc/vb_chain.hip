@@ -1,5 +1,10 @@
 // vb_chain.hip -- accept/reject of adaptive ETD attempts on the device.
 //
+// The shape-adjust loop (belastic.cpp:174-283) runs the same way:
+// k_shape_begin / k_shape_end carry its time, counters, step and energies, an
+// accepted candidate is copied over the state, and a rejection puts the
+// state's Euler prefix back from the copy taken before the update.
+//
 // A chain is a run of attempts issued back to back without the host: before
 // each attempt k_chain_begin opens the iteration (E0 and the dt doubling of
 // belastic.cpp:639-651) and picks the coefficient tables of the dt, which
@@ -77,9 +82,73 @@ __global__ void __launch_bounds__(64) k_chain_end(ChainCtl* c, Scal* sc, const S
   c->acc++;
 }
 
-// the copies that make an accepted candidate the state (nothing otherwise)
-__global__ void __launch_bounds__(256) k_chain_copy(const ChainCtl* c, ChainCopy cp) {
-  if (!c->accept) return;
+// x86 conversion of a double to int (truncation; INT_MIN out of range or NaN),
+// the host's, so that the check of the shape-adjust loop agrees with it
+__device__ __forceinline__ int trunc_int(double x) {
+  return (x < 2147483648.0 && x > -2147483649.0) ? int(x) : int(0x80000000u);
+}
+
+// Head of an iteration of the shape-adjust loop (belastic.cpp:196-236), when
+// none is open, and the attempt's step: the energy check every 1e-4 of time
+// (or every step once dt > 1e-4) ends the loop below 100 eta or at a relative
+// change under 2 %; dt doubles every tenth iteration.
+__global__ void __launch_bounds__(64) k_shape_begin(ShapeCtl* c, const Scal* sc) {
+  if (threadIdx.x != 0) return;
+  c->saved = *sc;
+  c->active = 0;
+  c->accept = 0;
+  if (c->status != SHAPE_RUN) return;
+  if (!c->inner_open) {
+    if (c->it >= 1000000) {
+      c->status = SHAPE_DONE;
+      return;
+    }
+    c->tcur += c->dt;
+    c->nst++;
+    const double dt = c->dt;
+    if (dt > 1e-4 || trunc_int(c->tcur / dt) % trunc_int(1e-4 / dt) == 0) {
+      const double eng1 = c->eng0;   // the state's energy (its prefix is the last accepted candidate's)
+      if (eng1 < 100 * c->eta || fabs((c->eng - eng1) / c->eng) < 0.02) {
+        c->status = SHAPE_DONE;
+        return;
+      }
+      c->eng = eng1;
+    }
+    if (c->nst % 10 == 0) c->dt = c->dt * 2;
+    c->inner_open = 1;
+  }
+  c->tstep = -c->dt * 3;
+  c->active = 1;
+}
+
+// The attempt's energy test (belastic.cpp:238-262): accepted below the
+// state's energy; otherwise dt halves, and the loop ends below 1e-13.  An
+// accepted non-finite state ends it too.
+__global__ void __launch_bounds__(64) k_shape_end(ShapeCtl* c, Scal* sc) {
+  if (threadIdx.x != 0) return;
+  if (!c->active) {
+    *sc = c->saved;
+    return;
+  }
+  c->evals++;
+  const double eng1 = sc->Eu;
+  if (eng1 < c->eng0) {
+    c->eng0 = eng1;
+    c->accept = 1;
+    c->inner_open = 0;
+    c->it++;
+    if (sc->nonfinite) c->status = SHAPE_DONE;
+    return;
+  }
+  *sc = c->saved;
+  c->rej++;
+  c->dt = c->dt / 2;
+  if (c->dt < 1e-13) c->status = SHAPE_DONE;
+}
+
+// copies of up to three buffers when *flag == want
+__global__ void __launch_bounds__(256) k_cond_copy(const int* flag, int want, ChainCopy cp) {
+  if (*flag != want) return;
   const long long stride = (long long)gridDim.x * 256;
   for (int j = 0; j < cp.n; ++j) {
     const v4u* src = static_cast<const v4u*>(cp.job[j].src);
@@ -129,14 +198,26 @@ void launch_chain_end(ChainCtl* c, Scal* sc, const Scal* test, hipStream_t s) {
   hipLaunchKernelGGL(k_chain_end, dim3(1), dim3(64), 0, s, c, sc, test);
 }
 
-void launch_chain_copy(const ChainCtl* c, const ChainCopy& cp, hipStream_t s) {
+void launch_cond_copy(const int* flag, int want, const ChainCopy& cp, hipStream_t s) {
   size_t most = 0;
   for (int j = 0; j < cp.n; ++j) {
     if (cp.job[j].bytes % 8 != 0) throw HipError{hipErrorInvalidValue, "chain copy: size not a multiple of 8 bytes"};
     most = std::max(most, cp.job[j].bytes / 16);
   }
   const unsigned blocks = unsigned(std::min<size_t>(4096, std::max<size_t>(1, (most + 1023) / 1024)));
-  hipLaunchKernelGGL(k_chain_copy, dim3(blocks), dim3(256), 0, s, c, cp);
+  hipLaunchKernelGGL(k_cond_copy, dim3(blocks), dim3(256), 0, s, flag, want, cp);
+}
+
+void launch_chain_copy(const ChainCtl* c, const ChainCopy& cp, hipStream_t s) {
+  launch_cond_copy(&c->accept, 1, cp, s);
+}
+
+void launch_shape_begin(ShapeCtl* c, const Scal* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_shape_begin, dim3(1), dim3(64), 0, s, c, sc);
+}
+
+void launch_shape_end(ShapeCtl* c, Scal* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_shape_end, dim3(1), dim3(64), 0, s, c, sc);
 }
 
 }  // namespace vb

@@ -82,6 +82,24 @@ struct ChainCtl {
   Scal saved;             // integrals of the state, put back after a rejection
 };
 
+// Control block of a chain of shape-adjust attempts decided on the device
+// (belastic.cpp:174-283): the loop's time, iteration count, step and energies,
+// carried by k_shape_begin / k_shape_end between the host's writes and reads.
+enum { SHAPE_RUN = 0, SHAPE_DONE = 1 };
+struct ShapeCtl {
+  double tcur, dt;        // the loop's time and step
+  double eng, eng0;       // energy at the last check; energy of the state
+  double eta;             // the stop level is 100 eta
+  double tstep;           // the update's factor of this attempt, -3 dt (read by k_xe)
+  int nst, it;            // the loop's counters
+  int inner_open;         // the current iteration's check and doubling are done
+  int status;             // SHAPE_*
+  int active, accept;     // the current attempt runs; it was accepted
+  int one, pad;           // one = 1: the condition of unconditional copies
+  int evals, rej;
+  Scal saved;             // integrals of the state, put back after a rejection
+};
+
 // ETD coefficient record, indexed by the integer m = kx^2 + ky'^2 + kz'^2
 // (folded indices).  ETDRK4: {e1, (e1-1)/H, e2, A s, 2 B s, C s, -, H} with
 // s = 1/(h^2 H^3).  ETDRK2: {e, (e-1)/H, c2, -, -, -, -, H}.

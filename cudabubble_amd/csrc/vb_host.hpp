@@ -98,6 +98,10 @@ void launch_chain_begin(ChainCtl* c, const Scal* sc, hipStream_t s);
 void launch_chain_table(const ChainCtl* c, double2* c1, double4* c2, int nrec, hipStream_t s);
 void launch_chain_end(ChainCtl* c, Scal* sc, const Scal* test, hipStream_t s);
 void launch_chain_copy(const ChainCtl* c, const ChainCopy& cp, hipStream_t s);
+// the copies of cp when *flag == want (device flag, read by the kernel)
+void launch_cond_copy(const int* flag, int want, const ChainCopy& cp, hipStream_t s);
+void launch_shape_begin(ShapeCtl* c, const Scal* sc, hipStream_t s);
+void launch_shape_end(ShapeCtl* c, Scal* sc, hipStream_t s);
 // *d = v on stream s (a scalar a captured graph reads)
 void launch_set_double(double* d, double v, hipStream_t s);
 

@@ -14,6 +14,7 @@
 // prefix(1) suffix(1) ... prefix(4) suffix(4), where prefix(1) of the next
 // evaluation is also the energy evaluation of the new state.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -241,7 +242,7 @@ struct Solver final : SolverBase {
              dt == o.dt;
     }
   };
-  enum { kFixedPair = 0, kAttempt = 1, kChain = 2, kShapeAttempt = 3, kEulerAttempt = 4 };
+  enum { kFixedPair = 0, kAttempt = 1, kChain = 2, kShapeAttempt = 3, kEulerAttempt = 4, kShapeChain = 5 };
   struct StepGraph {
     hipGraphExec_t x = nullptr;
     int kind = kFixedPair;
@@ -356,6 +357,7 @@ struct Solver final : SolverBase {
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (lut_stage) (void)hipHostFree(lut_stage);
     if (hchain) (void)hipHostFree(hchain);
+    if (hshape) (void)hipHostFree(hshape);
     flush_prof();
     for (auto e : evpool) (void)hipEventDestroy(e);
     if (hs) (void)hipHostFree(hs);
@@ -651,8 +653,8 @@ struct Solver final : SolverBase {
   }
 
   // Forward-Euler update of uh[cur] into uh[1-cur] with rhs g0 (shape) or nld1;
-  // tdev: the step t is read from euler_t (set by the caller)
-  int euler(double t, bool shape, bool tdev = false) {
+  // tdev non-null: the step is read from there on the device instead of t
+  int euler(double t, bool shape, const double* tdev = nullptr) {
     ensure_prefixed(true);
     const int out = 1 - cur;
     XeArgs<T> e;
@@ -670,7 +672,7 @@ struct Solver final : SolverBase {
     e.shape = shape ? 1 : 0;
     e.rl = Rows{g.ny, g.pitch, g.pp};
     e.part = nullptr;
-    e.tp = tdev ? euler_t.p : nullptr;
+    e.tp = tdev;
     L("xe", [&] { ox->xe(e, st); });
     L("y_fwd", [&] { ox->ypass(W.p, nullptr, 1, -1, g, twx.p, nullptr, st); });
     L("z_fwd", [&] { oz->zpass(W.p, -1, g, twz.p, st); });
@@ -732,7 +734,7 @@ struct Solver final : SolverBase {
       VB_HIP(hipGraphExecDestroy(G.x));
       G.x = nullptr;
     }
-    if (kind != kShapeAttempt && kind != kEulerAttempt)
+    if (kind != kShapeAttempt && kind != kEulerAttempt && kind != kShapeChain)
       (void)lut(p.scheme == VB_ETDRK2 ? 2 : 4, time_step);   // table upload happens outside the capture
     G.kind = kind;
     G.pre = pipe_state();
@@ -1056,10 +1058,12 @@ struct Solver final : SolverBase {
     return K;
   }
 
-  StepGraph* find_chain(int K) {
+  // a captured chain of kind (kChain, kShapeChain) and K attempts for the
+  // current pipeline state, whatever dt
+  StepGraph* find_chain(int K, int kind = kChain) {
     const PipeState now = pipe_state();
     for (auto& G : graphs)
-      if (G.x && G.kind == kChain && G.evals == K && G.pre.cur == now.cur && G.pre.prepped == now.prepped &&
+      if (G.x && G.kind == kind && G.evals == K && G.pre.cur == now.cur && G.pre.prepped == now.prepped &&
           G.pre.prefixed == now.prefixed && G.pre.prefixed_f == now.prefixed_f)
         return &G;
     return nullptr;
@@ -1207,7 +1211,7 @@ struct Solver final : SolverBase {
   DBuf<double> euler_t;
   void euler_attempt(double t, bool shape) {
     auto body = [&](bool tdev) {
-      const int out = euler(t, shape, tdev);
+      const int out = euler(t, shape, tdev ? euler_t.p : nullptr);
       prep(uh[out].p);
       cur = out;
       prefix(FIN_PLAIN, true);
@@ -1230,6 +1234,89 @@ struct Solver final : SolverBase {
     set_pipe_state(G->post);
   }
 
+  // The shape-adjust loop with its decisions on the device (accept_chain > 0
+  // with graphs): chains of kShapeSlots attempts, k_shape_begin / k_shape_end
+  // (vb_chain.hip) doing what shape_adjust() does on the host with the same
+  // doubles.  Before each update the state's Euler prefix (v, f, W) is copied
+  // aside; an accepted candidate is copied over the state (its prefix is then
+  // the state's), a rejected one, or an attempt after the loop ended, gets the
+  // prefix copied back.  The host reads the control block once per chain.
+  static constexpr int kShapeSlots = 8;
+  DBuf<ShapeCtl> shape_ctl;
+  ShapeCtl* hshape = nullptr;   // pinned
+  DBuf<T> v_keep, f_keep;
+  DBuf<cpx<T>> W_keep;
+
+  int shape_chain_body(int K) {
+    for (int k = 0; k < K; ++k) {
+      launch_shape_begin(shape_ctl.p, scal.p, st);
+      ChainCopy b{};
+      b.n = 3;
+      b.job[0] = {v.p, v_keep.p, v.n * sizeof(T)};
+      b.job[1] = {f.p, f_keep.p, f.n * sizeof(T)};
+      b.job[2] = {W.p, W_keep.p, W.n * sizeof(cpx<T>)};
+      launch_cond_copy(&shape_ctl.p->one, 1, b, st);
+      const int keep = cur;
+      const int out = euler(0.0, true, &shape_ctl.p->tstep);
+      prep(uh[out].p);
+      cur = out;
+      prefix(FIN_PLAIN, true);
+      cur = keep;
+      launch_shape_end(shape_ctl.p, scal.p, st);
+      ChainCopy a{};
+      a.n = 1;
+      a.job[0] = {uh[out].p, uh[keep].p, uh[keep].n * sizeof(cpx<double>)};
+      launch_cond_copy(&shape_ctl.p->accept, 1, a, st);
+      ChainCopy r{};
+      r.n = 3;
+      r.job[0] = {v_keep.p, v.p, v.n * sizeof(T)};
+      r.job[1] = {f_keep.p, f.p, f.n * sizeof(T)};
+      r.job[2] = {W_keep.p, W.p, W.n * sizeof(cpx<T>)};
+      launch_cond_copy(&shape_ctl.p->accept, 0, r, st);
+      check_launch();
+      prepped = false;
+      prefixed = false;
+      prefixed_f = true;
+    }
+    return K;
+  }
+
+  void shape_adjust_chain() {
+    ensure_prefixed(true);
+    read_scal();
+    if (!shape_ctl.p) {
+      shape_ctl.alloc(1);
+      v_keep.alloc(v.n);
+      f_keep.alloc(f.n);
+      W_keep.alloc(W.n);
+      VB_HIP(hipHostMalloc((void**)&hshape, sizeof(ShapeCtl), hipHostMallocDefault));
+    }
+    ShapeCtl& c = *hshape;
+    memset(&c, 0, sizeof(c));
+    c.tcur = 0.0;
+    c.dt = 1e-7;
+    c.eng = c.eng0 = hs->Eu;
+    c.eta = p.eta;
+    c.nst = 1;
+    c.status = SHAPE_RUN;
+    c.one = 1;
+    VB_HIP(hipMemcpyAsync(shape_ctl.p, hshape, sizeof(ShapeCtl), hipMemcpyHostToDevice, st));
+    StepGraph* G = find_chain(kShapeSlots, kShapeChain);
+    if (!G) G = &capture(kShapeChain, [&] { return shape_chain_body(kShapeSlots); });
+    for (;;) {
+      VB_HIP(hipGraphLaunch(G->x, st));
+      set_pipe_state(G->post);
+      VB_HIP(hipMemcpyAsync(hshape, shape_ctl.p, sizeof(ShapeCtl), hipMemcpyDeviceToHost, st));
+      VB_HIP(hipStreamSynchronize(st));
+      if (c.status != SHAPE_RUN) break;
+    }
+    hs_state = false;
+  }
+
+  // double to int as the x86 conversion does it (truncation; INT_MIN out of
+  // range or NaN), which k_shape_begin repeats on the device
+  static int trunc_int(double x) { return (x < 2147483648.0 && x > -2147483649.0) ? int(x) : INT_MIN; }
+
   // Forward-Euler shape relaxation, unconstrained (belastic.cpp:174-283)
   void shape_adjust() {
     double tcur = 0.0;
@@ -1241,7 +1328,7 @@ struct Solver final : SolverBase {
     for (int it = 0; it < 1000000; ++it) {
       tcur += dt;
       nst++;
-      if (dt > 1e-4 || int(tcur / dt) % int(1e-4 / dt) == 0) {
+      if (dt > 1e-4 || trunc_int(tcur / dt) % trunc_int(1e-4 / dt) == 0) {
         ensure_prefixed(true);
         read_scal();
         eng1 = hs->Eu;
@@ -1271,7 +1358,10 @@ struct Solver final : SolverBase {
   void start() {
     started = true;
     cur_E_known = false;
-    if (p.shape_adjust && time_current == 0.0) shape_adjust();
+    if (p.shape_adjust && time_current == 0.0) {
+      if (p.accept_chain > 0 && p.graphs && !prof) shape_adjust_chain();
+      else shape_adjust();
+    }
     ensure_prefixed(p.scheme == VB_EULER);
     if (beta <= 0.0) {   // belastic.cpp:588-591
       read_scal();

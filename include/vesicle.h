@@ -71,11 +71,13 @@ typedef struct vb_params {
                               attempts of shape_adjust and of the adaptive
                               VB_EULER driver, one graph for every dt;
                               0: eager launches only                            */
-  int accept_chain;        /* adaptive ETD with graphs: > 0: up to this many
-                              attempts (at most 16) per launch, accepted or
-                              rejected on the device with the host driver's
-                              arithmetic, one read-back per launch; 0: the host
-                              decides after each attempt (belastic.cpp:639-669) */
+  int accept_chain;        /* with graphs, > 0: adaptive ETD iterations run up
+                              to this many attempts (at most 16) per launch,
+                              accepted or rejected on the device with the host
+                              driver's arithmetic, one read-back per launch,
+                              and the shape_adjust pre-pass runs in chains of 8
+                              attempts the same way; 0: the host decides after
+                              each attempt (belastic.cpp:174-283, 639-669)     */
 } vb_params;
 
 /* Counters of one vb_step call. */

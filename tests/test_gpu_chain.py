@@ -124,3 +124,26 @@ def test_euler_attempt_graphs_match_eager(vb, gpu, prec):
                 "rejections", sum(r["rejections"] for r in out[0][1]))
     assert sum(r["rejections"] for r in out[0][1]) > 0   # the rejection path ran in graph form too
     assert np.array_equal(out[0][2], out[2][2]) and out[0][3] == out[2][3]
+
+
+@pytest.mark.parametrize("n,prec,kind", [(32, "FP64", 1), (32, "FP32_MIXED", 1), (64, "FP32_MIXED", 3)])
+def test_shape_adjust_chain_matches_host(vb, gpu, n, prec, kind):
+    # The shape-adjust pre-pass (belastic.cpp:174-283) decided on the device in
+    # chains of attempts: the same field as the host loop, bit for bit, and the
+    # same state for the steps after it (its pipeline state is the host's).
+    out = {}
+    for ac in (0, 8):
+        flush_print("shape adjust, accept_chain", ac, n, prec)
+        p, phi = make(vb, n, kind=kind, shape_adjust=1, M1=1e4, M2=1e4, alpha=-1.0, beta=-1.0, time_step=1e-6,
+                      fixed_time_step=1e-6, precision=getattr(vb, prec), accept_chain=ac)
+        s = vb.Solver(p, phi)
+        s.step(0)    # the driver start runs the shape adjustment
+        shaped, q = s.get_phi(), s.query()
+        r = s.step(3)
+        out[ac] = (shaped, q, r, s.get_phi(), s.state())
+        s.close()
+    a, b = out[0], out[8]
+    assert not np.array_equal(a[0], make(vb, n, kind=kind)[1])   # the pre-pass moved the field
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    assert same_report(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert same_report(a[4], b[4], a[4].keys())
