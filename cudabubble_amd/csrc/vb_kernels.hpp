@@ -41,8 +41,11 @@ template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 &&
 
 // Column tile width of the y passes and the z Laplacian (columns that are
 // contiguous in memory): enough columns for >= 128-byte segments and >= 256
-// threads.
+// threads, except FP32 at 64, 16 columns (128 threads): a 64^3 solve's y pass
+// is then 1056 workgroups instead of 528 (9.6 -> 7.7 us, a step 0.236 -> 0.209
+// ms; 32 columns at 32 changed nothing).
 template <typename T, int N> constexpr int colw() {
+  if (sizeof(T) == 4 && N == 64) return 16;
   constexpr int seg = 128 / int(sizeof(cpx<T>));
   constexpr int occ = 256 / TFof<N>();
   return seg > occ ? seg : occ;
@@ -56,8 +59,11 @@ template <typename T, int N> constexpr int colw() {
 // whose 128-register budget spilled 160-412 bytes per lane in the stage
 // combines; 8 took the combines from 1410-2358 to 1118-1440 us (the forward y
 // pass, which writes the 8-column groups, from 483 to 590 us) and a step from
-// 25.7 to 23.2 ms.
-template <typename T, int N> constexpr int tbw() { return (sizeof(T) == 4 && (N == 256 || N == 512)) ? 8 : colw<T, N>(); }
+// 25.7 to 23.2 ms.  FP32 at 64 keeps 32 (256 threads, the shared stage
+// launches' workgroup), though its y passes use 16 columns.
+template <typename T, int N> constexpr int tbw() {
+  return (sizeof(T) == 4 && (N == 256 || N == 512)) ? 8 : (sizeof(T) == 4 && N == 64) ? 32 : colw<T, N>();
+}
 // The other z tile a kernel schedule can select: the column width where tbw
 // is narrower, else half of it.
 template <typename T, int N> constexpr int tbw_alt() { return tbw<T, N>() < colw<T, N>() ? colw<T, N>() : tbw<T, N>() / 2; }
