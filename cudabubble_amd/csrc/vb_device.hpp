@@ -16,14 +16,23 @@ struct Geo {
   double kfac2;         // (2 pi / L)^2
   double L3;            // L^3
   // Row pitch of half spectra in elements.  pitch = nh is the reference
-  // layout; the solver pads rows to nx/2 + 16 so every row starts on a
-  // 128-byte boundary, and then (aligned = 1) the y and z kernels tile
-  // kx in [0, nx/2) with aligned column groups and treat the Nyquist column
-  // kx = nx/2 as a separate group.
+  // layout (nq = -1: the Nyquist column kx = nx/2 ends each row).  The solver
+  // (aligned = 1) keeps rows of kx in [0, nx/2) only, pitch = nx/2, so every
+  // row is whole 128-byte lines, and stores the Nyquist column of a plane as
+  // ny contiguous elements after its rows, at plane offset nq: the row kernels
+  // then read no line for one element, and the y and z kernels tile kx in
+  // [0, nx/2) with aligned column groups and read the Nyquist column as a
+  // separate, contiguous group.
   int pitch, aligned;
   long long pp;         // plane pitch (z stride) of spectra in elements
   long long ncp;        // allocated spectrum elements, nz * pp
+  long long nq;         // plane offset of the Nyquist column, or -1 (in the rows)
 };
+
+// Position of mode (z, ky, kx) of a half spectrum in the row layout.
+__host__ __device__ inline long long spec_pos(const Geo& g, int z, int ky, int kx) {
+  return (long long)z * g.pp + ((g.nq >= 0 && kx == g.nx / 2) ? g.nq + ky : (long long)ky * g.pitch + kx);
+}
 
 // Physical constants of one solve (reference names).
 struct Phys {

@@ -69,36 +69,44 @@ template <typename T, int N> constexpr int tbw() {
 template <typename T, int N> constexpr int tbw_alt() { return tbw<T, N>() < colw<T, N>() ? colw<T, N>() : tbw<T, N>() / 2; }
 
 // Z_p = A_ext(p) + i * B_ext(p): Hermitian extension along x of two half
-// spectra.  The imaginary parts at kx = 0 and kx = N/2 are dropped, which is
-// what a real-output inverse does with them.
+// spectra (rows A and B, whose Nyquist elements are at An and Bn).  The
+// imaginary parts at kx = 0 and kx = N/2 are dropped, which is what a
+// real-output inverse does with them.
 template <typename T, int N>
-__device__ __forceinline__ cpx<T> pack2(const cpx<T>* __restrict__ A, const cpx<T>* __restrict__ B, int p) {
+__device__ __forceinline__ cpx<T> pack2(const cpx<T>* A, const cpx<T>* An, const cpx<T>* B, const cpx<T>* Bn, int p) {
   const bool mir = p > N / 2;
   const int k = mir ? N - p : p;
-  cpx<T> a = A[k], b = B[k];
+  cpx<T> a = *(k == N / 2 ? An : A + k), b = *(k == N / 2 ? Bn : B + k);
   if (k == 0 || k == N / 2) { a.y = T(0); b.y = T(0); }
   if (mir) { a.y = -a.y; b.y = -b.y; }
   return mk<T>(a.x - b.y, a.y + b.x);
 }
 
-// Layout of half-spectrum rows: row r = z * ny + y starts at z * pp + y * pitch.
+// Layout of half-spectrum rows: row r = z * ny + y starts at z * pp + y * pitch;
+// its Nyquist element is at z * pp + nq + y (nq >= 0) or ends the row.
 // Rows number at most 512 * 512, so the division is a 32-bit one.
 struct Rows {
   int ny, pitch;
   long long pp;
+  long long nq;
   __host__ __device__ long long off(long long r) const {
     const unsigned q = unsigned(r) / unsigned(ny), m = unsigned(r) - q * unsigned(ny);
     return (long long)q * pp + (long long)m * pitch;
+  }
+  __host__ __device__ long long nyq(long long r, int half) const {
+    const unsigned q = unsigned(r) / unsigned(ny), m = unsigned(r) - q * unsigned(ny);
+    return (long long)q * pp + (nq >= 0 ? nq + m : (long long)m * pitch + half);
   }
 };
 
 // After a forward transform of Z = a + i b (a, b real rows), write the half
 // spectra of a and b, times s (and b's times sb as well); SCALE = false writes
 // them unscaled and doubled, Z_k + conj Z_{N-k} and -i (Z_k - conj Z_{N-k}).
-// Uses the lane's LDS; ends with a barrier.
+// Uses the lane's LDS; ends with a barrier.  outAn, outBn: where the rows'
+// Nyquist elements go.
 template <typename T, int N, int V = 0, bool SCALE = true, class Lds>
-__device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, cpx<T>* __restrict__ outA,
-                                             cpx<T>* __restrict__ outB, T s, bool valid, T sb = T(1)) {
+__device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, cpx<T>* outA, cpx<T>* outAn,
+                                             cpx<T>* outB, cpx<T>* outBn, T s, bool valid, T sb = T(1)) {
   constexpr int E = Plan<N, V>::E, TF = N / E, NH = N / 2 + 1;
 #pragma unroll
   for (int e = 0; e < E; ++e) lds(j + TF * e) = x[e];
@@ -111,12 +119,14 @@ __device__ __forceinline__ void unpack_store(const cpx<T>* x, int j, Lds& lds, c
       const cpx<T> zm = lds(k == 0 ? 0 : N - k);
       const T h = T(0.5) * s, hb = h * sb;
       // a_k = (Z_k + conj Z_{N-k}) / 2,  b_k = (Z_k - conj Z_{N-k}) / (2i)
+      cpx<T>* const oa = k == N / 2 ? outAn : outA + k;
+      cpx<T>* const ob = k == N / 2 ? outBn : outB + k;
       if constexpr (SCALE) {
-        outA[k] = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
-        outB[k] = mk<T>((zk.y + zm.y) * hb, (zm.x - zk.x) * hb);
+        *oa = mk<T>((zk.x + zm.x) * h, (zk.y - zm.y) * h);
+        *ob = mk<T>((zk.y + zm.y) * hb, (zm.x - zk.x) * hb);
       } else {
-        outA[k] = mk<T>(zk.x + zm.x, zk.y - zm.y);
-        outB[k] = mk<T>(zk.y + zm.y, zm.x - zk.x);
+        *oa = mk<T>(zk.x + zm.x, zk.y - zm.y);
+        *ob = mk<T>(zk.y + zm.y, zm.x - zk.x);
       }
     }
   }
@@ -166,8 +176,8 @@ __global__ void __launch_bounds__(XWG) k_xr2c(const TI* __restrict__ in, cpx<T>*
     x[e] = mk<T>(T(in[r0 * N + p]), T(in[(r0 + 1) * N + p]));
   }
   fft<T, N, -1, XP>(x, j, tw, lds);
-  const long long o0 = rl.off(r0);
-  unpack_store<T, N, XP>(x, j, lds, out + o0, out + o0 + rl.pitch, T(scl), valid);
+  unpack_store<T, N, XP>(x, j, lds, out + rl.off(r0), out + rl.nyq(r0, N / 2), out + rl.off(r0 + 1),
+                         out + rl.nyq(r0 + 1, N / 2), T(scl), valid);
 }
 
 template <typename T, typename TO, int N>
@@ -182,9 +192,12 @@ __global__ void __launch_bounds__(XWG) k_xc2r(const cpx<T>* __restrict__ in, TO*
   if (!valid) pair = npairs - 1;
   const long long r0 = 2 * pair;
   cpx<T> x[E];
-  const long long o0 = rl.off(r0);
+  const cpx<T>* const i0 = in + rl.off(r0);
+  const cpx<T>* const i1 = in + rl.off(r0 + 1);
+  const cpx<T>* const n0 = in + rl.nyq(r0, N / 2);
+  const cpx<T>* const n1 = in + rl.nyq(r0 + 1, N / 2);
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(in + o0, in + o0 + rl.pitch, j + TF * e);
+  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(i0, n0, i1, n1, j + TF * e);
   fft<T, N, +1, XP>(x, j, tw, lds);
   if (valid) {
 #pragma unroll
@@ -248,7 +261,8 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
     for (int h = 0; h < 2; ++h) {
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        x[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.Q + a.rl.off(r0 + h), j + TF * e);
+        x[h][e] = pack2<T, N>(a.P + a.rl.off(r0 + h), a.P + a.rl.nyq(r0 + h, N / 2), a.Q + a.rl.off(r0 + h),
+                              a.Q + a.rl.nyq(r0 + h, N / 2), j + TF * e);
     }
     fft2<T, N, +1, XP>(x[0], x[1], j, a.tw, lds, lds2);
 #pragma unroll
@@ -278,14 +292,16 @@ __global__ void __launch_bounds__(XWG, (xwaves<T>())) k_xa(XaArgs<T> a0, const X
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(tm[0][e], tm[1][e]);
     fft<T, N, -1, XP>(z, j, a.tw, lds);
-    unpack_store<T, N, XP>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    unpack_store<T, N, XP>(z, j, lds, a.Tt + a.rl.off(r0), a.Tt + a.rl.nyq(r0, N / 2), a.Tt + a.rl.off(r0 + 1),
+                           a.Tt + a.rl.nyq(r0 + 1, N / 2), T(a.inv_n), valid);
   }
   if constexpr (WITHF) {
     cpx<T> z[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) z[e] = mk<T>(ff[0][e], ff[1][e]);
     fft<T, N, -1, XP>(z, j, a.tw, lds);
-    unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+    unpack_store<T, N, XP>(z, j, lds, a.W + a.rl.off(r0), a.W + a.rl.nyq(r0, N / 2), a.W + a.rl.off(r0 + 1),
+                           a.W + a.rl.nyq(r0 + 1, N / 2), T(a.inv_n), valid);
   }
   block_sum_store<3, XWG>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
@@ -375,7 +391,9 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   const double D = a.D, e3 = eps * eps * eps;
   cpx<T> x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(row), a.Q + a.rl.off(row), j + TF * e);
+  for (int e = 0; e < E; ++e)
+    x[e] = pack2<T, N>(a.P + a.rl.off(row), a.P + a.rl.nyq(row, N / 2), a.Q + a.rl.off(row),
+                       a.Q + a.rl.nyq(row, N / 2), j + TF * e);
   fft<T, N, +1, XP>(x, j, a.tw, lds);   // v + i lap v
   double acc[3] = {0.0, 0.0, 0.0};
   // FP32: the k = 0 coefficients of tmp1 and eps^3 s are sums over the whole
@@ -421,7 +439,8 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
   if constexpr (DCS && NT == 64) {
     // one wave: the five sums in one exchange (wave_sum_store) after the transform
     fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
-    unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(1), valid);
+    unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Tt + a.rl.nyq(row, N / 2), a.Ts + a.rl.off(row),
+                                  a.Ts + a.rl.nyq(row, N / 2), T(1), valid);
     __shared__ double wred[5 * 64];
     const double v5[5] = {acc[0], acc[1], acc[2], dcs[0], dcs[1]};
     double* const o3 = a.part + 3 * (long long)blockIdx.x;
@@ -436,7 +455,8 @@ __global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, 
     }
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);   // tmp1 + i s
-  unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Ts + a.rl.off(row), T(1), valid);
+  unpack_store<T, N, XP, false>(x, j, lds, a.Tt + a.rl.off(row), a.Tt + a.rl.nyq(row, N / 2), a.Ts + a.rl.off(row),
+                                a.Ts + a.rl.nyq(row, N / 2), T(1), valid);
   block_sum_store<3, NT>(acc, red, a.part + 3 * (long long)blockIdx.x);
 }
 
@@ -466,7 +486,9 @@ __global__ void __launch_bounds__(XWG) k_xsums(XaArgs<T> a0, const XaArgs<T>* ta
     const long long r0 = 2 * pair;
     cpx<T> x[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.P + a.rl.off(r0), a.P + a.rl.off(r0 + 1), j + TF * e);
+    for (int e = 0; e < E; ++e)
+      x[e] = pack2<T, N>(a.P + a.rl.off(r0), a.P + a.rl.nyq(r0, N / 2), a.P + a.rl.off(r0 + 1),
+                         a.P + a.rl.nyq(r0 + 1, N / 2), j + TF * e);
     fft<T, N, +1, XP>(x, j, a.tw, lds);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -512,7 +534,9 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
   const long long r0 = 2 * pair;
   cpx<T> x[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.off(r0) + a.rl.pitch, j + TF * e);
+  for (int e = 0; e < E; ++e)
+    x[e] = pack2<T, N>(a.W + a.rl.off(r0), a.W + a.rl.nyq(r0, N / 2), a.W + a.rl.off(r0 + 1),
+                       a.W + a.rl.nyq(r0 + 1, N / 2), j + TF * e);
   fft<T, N, +1, XP>(x, j, a.tw, lds);
   const double l1 = a.sc->l1, l2 = a.sc->l2, eta = a.eta, eps = a.eps, t = a.tp ? *a.tp : a.t, ie2 = 1.0 / (eps * eps);
   double acc[1] = {0.0};
@@ -538,16 +562,19 @@ __global__ void __launch_bounds__(XWG) k_xe(XeArgs<T> a) {
     return;
   }
   fft<T, N, -1, XP>(x, j, a.tw, lds);
-  unpack_store<T, N, XP>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.off(r0) + a.rl.pitch, T(a.inv_n), valid);
+  unpack_store<T, N, XP>(x, j, lds, a.out + a.rl.off(r0), a.out + a.rl.nyq(r0, N / 2), a.out + a.rl.off(r0 + 1),
+                         a.out + a.rl.nyq(r0 + 1, N / 2), T(a.inv_n), valid);
 }
 
 // ---------------------------------------------------------------------------
 // y kernels: complex transforms along y, in place, batch over (z, kx).
 // blockIdx.y selects one of two arrays.
 // ---------------------------------------------------------------------------
-// Column of one y transform: (z, kx) -> base address; rows are pitch apart.
+// Column of one y transform: (z, kx) -> base address and the stride between
+// its ky (pitch; 1 in a plane's Nyquist column, see Geo).
 struct YIdx {
-  long long base;   // z * pp + kx
+  long long base;
+  long long stride;
   int z, kx;
   bool valid;
 };
@@ -582,7 +609,9 @@ __device__ __forceinline__ YIdx yidx(const Geo& g, long long t) {
     z = int(b / g.nh);
     kx = int(b % g.nh);
   }
-  r.base = (long long)z * g.pp + kx;
+  const bool seg = g.nq >= 0 && kx == g.nx / 2;
+  r.base = (long long)z * g.pp + (seg ? g.nq : kx);
+  r.stride = seg ? 1 : g.pitch;
   r.z = z;
   r.kx = kx;
   return r;
@@ -657,7 +686,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     // the forward pass into the group-major stage inputs is the last reader of P, Q
-    const cpx<T>* src = col + (long long)(j + TF * e) * g.pitch;
+    const cpx<T>* src = col + (long long)(j + TF * e) * id.stride;
     x[e] = GM ? ld_nt(src) : *src;
   }
   fft<T, N, S, V>(x, j, tw, lds);
@@ -676,7 +705,7 @@ __global__ void __launch_bounds__((TB * TFof<N, V>())) k_ypass(YArgs<T> a0, cons
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * g.pitch] = x[e];
+      for (int e = 0; e < E; ++e) col[(long long)(j + TF * e) * id.stride] = x[e];
     }
   }
 }
@@ -717,7 +746,7 @@ __device__ __forceinline__ ZIdx zidx(const Geo& g, long long t, int c) {
     r.ky = int(q / g.nh);
     r.kx = int(q % g.nh);
   }
-  r.q = (long long)r.ky * g.pitch + r.kx;
+  r.q = (g.nq >= 0 && r.kx == g.nx / 2) ? g.nq + r.ky : (long long)r.ky * g.pitch + r.kx;
   return r;
 }
 template <int TB>
@@ -1170,7 +1199,7 @@ __global__ void k_relayout(RelArgs<TI> a0, const RelArgs<TI>* tab) {
   const int kx = int(i % (g.nx / 2 + 1));
   const int ky = int((i / (g.nx / 2 + 1)) % g.ny);
   const int z = int(i / ((long long)(g.nx / 2 + 1) * g.ny));
-  const long long rw = (long long)z * g.pp + (long long)ky * g.pitch + kx;
+  const long long rw = spec_pos(g, z, ky, kx);
   const cpx<TI> v = a.in[a.tb_in ? gm_index(g, a.tb_in, z, ky, kx) : rw];
   a.out[a.tb_out ? gm_index(g, a.tb_out, z, ky, kx) : rw] = mk<double>(double(v.x), double(v.y));
 }

@@ -135,6 +135,7 @@ Geo make_geo(int nx, int nz, double L) {
   g.aligned = 0;
   g.pp = (long long)g.ny * g.nh;
   g.ncp = g.nc;
+  g.nq = -1;
   return g;
 }
 

@@ -34,7 +34,7 @@ struct Launch {
 
   static unsigned xgrid(long long npairs) { return unsigned((npairs + G - 1) / G); }
 
-  static Rows rows(const Geo& g) { return Rows{g.ny, g.pitch, g.pp}; }
+  static Rows rows(const Geo& g) { return Rows{g.ny, g.pitch, g.pp, g.nq}; }
   static void xr2c_d(const double* in, cpx<T>* out, long long np, const cpx<T>* tw, double scl, const Geo& g,
                      hipStream_t s) {
     hipLaunchKernelGGL((k_xr2c<double, T, N>), dim3(xgrid(np)), dim3(XWG), 0, s, in, out, np, tw, scl, rows(g));

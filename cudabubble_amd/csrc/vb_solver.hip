@@ -276,11 +276,13 @@ struct Solver final : SolverBase {
       tiles = tiles && (p.nx / 2) % t == 0;
     }
     if ((p.nx / 2) % 16 == 0 && tiles) {
-      g.pitch = p.nx / 2 + 16;
+      // rows of nx/2 modes (whole 128-byte lines), then the plane's Nyquist
+      // column; one more aligned unit of 16 keeps the z stride away from large
+      // powers of two (which concentrate z columns on few HBM channels)
+      g.pitch = p.nx / 2;
       g.aligned = 1;
-      // one extra aligned row unit per plane keeps the z stride away from
-      // large powers of two (which concentrate z columns on few HBM channels)
-      g.pp = (long long)g.ny * g.pitch + 16;
+      g.nq = (long long)g.ny * g.pitch;
+      g.pp = g.nq + g.ny + 16;
       g.ncp = (long long)g.nz * g.pp;
     }
   }
@@ -541,7 +543,7 @@ struct Solver final : SolverBase {
     a.kappa = p.kappa;
     a.D = p.D;
     a.inv_n = 1.0 / double(g.nr);
-    a.rl = Rows{g.ny, g.pitch, g.pp};
+    a.rl = Rows{g.ny, g.pitch, g.pp, g.nq};
     L(withf ? "xa_f" : "xa", [&] { ox->xa(a, withf ? XA_WITHF : XA_WITHS, st); });
     // the integrals of this field (xa's and the z kernel's partials) are
     // evaluated by extra workgroups of the next column pass
@@ -573,7 +575,7 @@ struct Solver final : SolverBase {
     a.kappa = p.kappa;
     a.D = p.D;
     a.inv_n = 1.0 / double(g.nr);
-    a.rl = Rows{g.ny, g.pitch, g.pp};
+    a.rl = Rows{g.ny, g.pitch, g.pp, g.nq};
     L("xa_sums", [&] { ox->xa(a, XA_SUMS, st); });
     refinalize(mode, (nxb + kXsLoop - 1) / kXsLoop);
     prepped = false;
@@ -670,7 +672,7 @@ struct Solver final : SolverBase {
     e.t = t;
     e.inv_n = 1.0 / double(g.nr);
     e.shape = shape ? 1 : 0;
-    e.rl = Rows{g.ny, g.pitch, g.pp};
+    e.rl = Rows{g.ny, g.pitch, g.pp, g.nq};
     e.part = nullptr;
     e.tp = tdev;
     L("xe", [&] { ox->xe(e, st); });
@@ -1499,7 +1501,7 @@ struct Solver final : SolverBase {
     a.kappa = p.kappa;
     a.D = p.D;
     a.inv_n = 1.0 / double(g.nr);
-    a.rl = Rows{g.ny, g.pitch, g.pp};
+    a.rl = Rows{g.ny, g.pitch, g.pp, g.nq};
     L("xsums64", [&] { oxd->xa(a, XA_ROWSUMS, st); });
     launch_keep(OP_KEEP, scal.p, scal64.p, ph, st);   // the targets
     L("finalize64", [&] { launch_finalize(xpart64.p, nxb64, zpart64.p, dc64.p, nzb64, scal64.p, g, ph, FIN_PLAIN, st); });
@@ -1572,28 +1574,36 @@ struct Solver final : SolverBase {
   }
 
   // the state spectrum in the reference layout [nz][ny][nx/2+1] (the
-  // solver's rows are padded to g.pitch, planes to g.pp)
+  // solver's row layout: spec_pos)
   void get_spec(double* host) override {
     L("to_rows", [&] { relayout(uh[cur].p, oz->ytile, S.p, 0); });
     check_launch();
     std::vector<cpx<double>> h((size_t)g.ncp);
     VB_HIP(hipMemcpyAsync(h.data(), S.p, sizeof(cpx<double>) * h.size(), hipMemcpyDeviceToHost, st));
     VB_HIP(hipStreamSynchronize(st));
-    const size_t nh = (size_t)g.nx / 2 + 1;
-    for (size_t z = 0; z < (size_t)g.nz; ++z)
-      for (size_t y = 0; y < (size_t)g.ny; ++y)
-        memcpy(host + 2 * ((z * g.ny + y) * nh), &h[z * g.pp + y * g.pitch], sizeof(cpx<double>) * nh);
+    const int nh = g.nx / 2 + 1;
+    for (int z = 0; z < g.nz; ++z)
+      for (int y = 0; y < g.ny; ++y)
+        for (int k = 0; k < nh; ++k) {
+          const cpx<double>& v = h[(size_t)spec_pos(g, z, y, k)];
+          double* o = host + 2 * (((size_t)z * g.ny + y) * nh + k);
+          o[0] = v.x;
+          o[1] = v.y;
+        }
   }
 
   void set_spec(const double* host) override {
     cur_E_known = false;
     hs_state = false;
     iter_open = false;
-    const size_t nh = (size_t)g.nx / 2 + 1;
+    const int nh = g.nx / 2 + 1;
     std::vector<cpx<double>> h((size_t)g.ncp, cpx<double>{0.0, 0.0});
-    for (size_t z = 0; z < (size_t)g.nz; ++z)
-      for (size_t y = 0; y < (size_t)g.ny; ++y)
-        memcpy(&h[z * g.pp + y * g.pitch], host + 2 * ((z * g.ny + y) * nh), sizeof(cpx<double>) * nh);
+    for (int z = 0; z < g.nz; ++z)
+      for (int y = 0; y < g.ny; ++y)
+        for (int k = 0; k < nh; ++k) {
+          const double* i = host + 2 * (((size_t)z * g.ny + y) * nh + k);
+          h[(size_t)spec_pos(g, z, y, k)] = cpx<double>{i[0], i[1]};
+        }
     VB_HIP(hipMemcpyAsync(S.p, h.data(), sizeof(cpx<double>) * h.size(), hipMemcpyHostToDevice, st));
     L("to_groups", [&] { relayout(S.p, 0, uh[cur].p, oz->ytile); });
     check_launch();
@@ -1623,7 +1633,7 @@ struct Solver final : SolverBase {
     e.t = 0.0;
     e.inv_n = 1.0 / double(g.nr);
     e.shape = 0;
-    e.rl = Rows{g.ny, g.pitch, g.pp};
+    e.rl = Rows{g.ny, g.pitch, g.pp, g.nq};
     e.part = rpart.p;
     e.tp = nullptr;
     L("xe_resid", [&] { ox->xe(e, st); });
