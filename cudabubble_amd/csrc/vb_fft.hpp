@@ -6,8 +6,9 @@
 // its inputs at exactly those positions, so the first pass takes its data
 // straight from global memory, the last pass leaves its results in the same
 // positions (ready for coalesced stores), and the passes in between exchange
-// data through LDS.  Radices are 2, 3, 4 and 8; the radix plan per length is
-// fixed at compile time so every loop unrolls.
+// data through LDS.  Radices are 2, 3, 4, 8, 16 (4 x 4 in registers) and 24
+// (3 x 8); the radix plan per length is fixed at compile time so every loop
+// unrolls.
 //
 // Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host
 // in extended precision and rounded once to the storage type.
@@ -48,8 +49,9 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a) {
 // ---------------------------------------------------------------------------
 // V = 0 is the plan of every kernel.  V = 1 trades registers for fewer
 // Stockham passes (fewer LDS exchanges and workgroup barriers); it differs
-// only where measured to pay: 256 = 16 * 16 and 128 = 16 * 8 for the FP32
-// column kernels and x kernels.
+// only where measured to pay: 256 = 16 * 16, 128 = 16 * 8 and 192 = 24 * 8
+// for the FP32 column kernels and x kernels, 512 = 16 * 8 * 4 for the FP32 x
+// kernel.
 // V = 2 goes the other way (more threads per transform, fewer registers).
 template <int N, int V = 0> struct Plan;
 template <> struct Plan<16, 0>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
@@ -66,6 +68,7 @@ template <int N> struct Plan<N, 1> : Plan<N, 0> {};
 template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 16}; };
 template <> struct Plan<128, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 8}; };
 template <> struct Plan<512, 1> { static constexpr int E = 16, NP = 3; static constexpr int R[3] = {16, 8, 4}; };
+template <> struct Plan<192, 1> { static constexpr int E = 24, NP = 2; static constexpr int R[2] = {24, 8}; };
 
 template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 
@@ -163,6 +166,44 @@ template <int S, typename T> struct Dft<16, S, T> {
       Dft<4, S, T>::run(t);
 #pragma unroll
       for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
+// 24 points as 3 x 8: n = n2 + 3 n1, k = k1 + 8 k2, so
+// X[k1 + 8 k2] = sum_n2 W3^(n2 k2) W24^(n2 k1) sum_n1 x[n2 + 3 n1] W8^(n1 k1).
+template <int S, typename T> struct Dft<24, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    // cos and sin of 2 pi e / 24 (multiples of 15 degrees), e = 0..14
+    constexpr double c15 = 0.96592582628906828675, s15 = 0.25881904510252076235;
+    constexpr double c30 = 0.86602540378443864676, c45 = 0.70710678118654752440;
+    constexpr double cs[15][2] = {{1, 0},        {c15, s15},   {c30, 0.5},  {c45, c45},   {0.5, c30},
+                                  {s15, c15},    {0, 1},       {-s15, c15}, {-0.5, c30},  {-c45, c45},
+                                  {-c30, 0.5},   {-c15, s15},  {-1, 0},     {-c15, -s15}, {-c30, -0.5}};
+    cpx<T> a[3][8];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      cpx<T> t[8];
+#pragma unroll
+      for (int n1 = 0; n1 < 8; ++n1) t[n1] = v[n2 + 3 * n1];
+      Dft<8, S, T>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) a[n2][k1] = t[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 3; ++n2) {
+#pragma unroll
+      for (int k1 = 1; k1 < 8; ++k1) {
+        const int e = n2 * k1;
+        a[n2][k1] = cmul(a[n2][k1], mk<T>(T(cs[e][0]), T(S * cs[e][1])));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) {
+      cpx<T> t[3] = {a[0][k1], a[1][k1], a[2][k1]};
+      Dft<3, S, T>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) v[k1 + 8 * k2] = t[k2];
     }
   }
 };

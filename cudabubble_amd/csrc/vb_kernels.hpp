@@ -34,10 +34,13 @@ template <typename T, int N> constexpr int xlanes() { return XWG / TFof<N, xplan
 // gains more in latency hiding than it loses); FP64 keeps the full budget.
 template <typename T> constexpr int xwaves() { return sizeof(T) == 4 ? 4 : 1; }
 
-// Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256 and
-// 128: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>), 16 x 8 instead of
-// 8 x 4 x 4 (with the x kernel's, 12 x 128^3 batched 3.74 -> 3.67 ms per step).
-template <typename T, int N> constexpr int colplan() { return (sizeof(T) == 4 && (N == 256 || N == 128)) ? 1 : 0; }
+// Plan variant of the FP32 column kernels (y passes, z Laplacian) at 256, 128
+// and 192: 16 x 16 instead of 8 x 8 x 4 (see Plan<256, 1>), 16 x 8 instead of
+// 8 x 4 x 4 (with the x kernel's, 12 x 128^3 batched 3.74 -> 3.67 ms per step),
+// 24 x 8 instead of 4 x 4 x 4 x 3 (forward y pass 32.2 -> 26.7 us at 192^3).
+template <typename T, int N> constexpr int colplan() {
+  return (sizeof(T) == 4 && (N == 256 || N == 128 || N == 192)) ? 1 : 0;
+}
 
 // Column tile width of the y passes and the z Laplacian (columns that are
 // contiguous in memory): enough columns for >= 128-byte segments and >= 256
@@ -339,9 +342,11 @@ __device__ __forceinline__ void wave_sum_store(const double (&v)[K], double* sm,
 // SIMD, against 116 and four for 8 x 8 x 4) and raised the FP64 rate 3 % (250
 // registers, two waves); at 128 in FP32 the 16 x 8 plan (one exchange instead
 // of two); at 512 in FP32 16 x 8 x 4 (half the threads per row of 8 x 8 x 8,
-// 760 -> 688 us).  The FP64 x kernel at 128 keeps 8 x 4 x 4.
+// 760 -> 688 us); at 192 in FP32 24 x 8 (one exchange instead of three; 191
+// registers and two waves per SIMD against 132 and three, 38.5 -> 33.8 us at
+// 192^3).  The FP64 x kernel at 128 keeps 8 x 4 x 4.
 template <typename T, int N> constexpr int xrplan() {
-  return (N == 256 || ((N == 128 || N == 512) && sizeof(T) == 4)) ? 1 : xplan<T, N>();
+  return (N == 256 || ((N == 128 || N == 192 || N == 512) && sizeof(T) == 4)) ? 1 : xplan<T, N>();
 }
 // the other plan a kernel schedule can select (the same where no two-pass
 // plan exists)

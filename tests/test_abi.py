@@ -191,7 +191,10 @@ def test_schedules_without_gpu(vb):
     # kernel schedules per grid length (vb_schedule_*): host-side tables only
     assert vb.schedule_count(256, vb.FP32_MIXED) == 4
     assert vb.schedule_count(128, vb.FP32_MIXED) == 4
-    assert vb.schedule_count(192, vb.FP32_MIXED) == 2
+    assert vb.schedule_count(192, vb.FP32_MIXED) == 4
+    # at 192 the default x plan is 24 x 8 (one LDS exchange), the other 4 x 4 x 4 x 3
+    assert vb.schedule_describe(192, vb.FP32_MIXED, 0) == {"z_tile": 16, "x_passes": 2, "x_radix": 24}
+    assert vb.schedule_describe(192, vb.FP32_MIXED, 2) == {"z_tile": 16, "x_passes": 4, "x_radix": 4}
     assert vb.schedule_count(256, vb.FP64) == 1
     assert vb.schedule_count(100, vb.FP32_MIXED) == 0 and vb.schedule_get(100, vb.FP32_MIXED) == -1
     d = [vb.schedule_describe(256, vb.FP32_MIXED, s) for s in range(4)]
