@@ -6,9 +6,9 @@
 // its inputs at exactly those positions, so the first pass takes its data
 // straight from global memory, the last pass leaves its results in the same
 // positions (ready for coalesced stores), and the passes in between exchange
-// data through LDS.  Radices are 2, 3, 4, 8, 16 (4 x 4 in registers) and 24
-// (3 x 8); the radix plan per length is fixed at compile time so every loop
-// unrolls.
+// data through LDS.  Radices are 2, 3, 4, 8, 12 (3 x 4 in registers), 16
+// (4 x 4) and 24 (3 x 8); the radix plan per length is fixed at compile time
+// so every loop unrolls.
 //
 // Twiddles come from a table tw[m] = exp(-2 pi i m / N) computed on the host
 // in extended precision and rounded once to the storage type.
@@ -53,6 +53,8 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a) {
 // for the FP32 column kernels and x kernels, 512 = 16 * 8 * 4 for the FP32 x
 // kernel.
 // V = 2 goes the other way (more threads per transform, fewer registers).
+// V = 3 keeps V = 0's elements per thread with fewer passes (192 = 12 * 4 * 4,
+// the FP32 stage combines).
 template <int N, int V = 0> struct Plan;
 template <> struct Plan<16, 0>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
 template <> struct Plan<32, 0>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 4}; };
@@ -69,6 +71,7 @@ template <> struct Plan<256, 1> { static constexpr int E = 16, NP = 2; static co
 template <> struct Plan<128, 1> { static constexpr int E = 16, NP = 2; static constexpr int R[2] = {16, 8}; };
 template <> struct Plan<512, 1> { static constexpr int E = 16, NP = 3; static constexpr int R[3] = {16, 8, 4}; };
 template <> struct Plan<192, 1> { static constexpr int E = 24, NP = 2; static constexpr int R[2] = {24, 8}; };
+template <> struct Plan<192, 3> { static constexpr int E = 12, NP = 3; static constexpr int R[3] = {12, 4, 4}; };
 
 template <int N, int V = 0> constexpr int TFof() { return N / Plan<N, V>::E; }
 
@@ -166,6 +169,39 @@ template <int S, typename T> struct Dft<16, S, T> {
       Dft<4, S, T>::run(t);
 #pragma unroll
       for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
+// 12 points as 3 x 4: n = n2 + 3 n1, k = k1 + 4 k2, so
+// X[k1 + 4 k2] = sum_n2 W3^(n2 k2) W12^(n2 k1) sum_n1 x[n2 + 3 n1] W4^(n1 k1).
+template <int S, typename T> struct Dft<12, S, T> {
+  static __device__ __forceinline__ void run(cpx<T>* v) {
+    // cos and sin of 2 pi e / 12 (multiples of 30 degrees), e = 0..6
+    constexpr double c30 = 0.86602540378443864676;
+    constexpr double cs[7][2] = {{1, 0}, {c30, 0.5}, {0.5, c30}, {0, 1}, {-0.5, c30}, {-c30, 0.5}, {-1, 0}};
+    cpx<T> a[3][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      cpx<T> t[4] = {v[n2], v[n2 + 3], v[n2 + 6], v[n2 + 9]};
+      Dft<4, S, T>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 3; ++n2) {
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) {
+        const int e = n2 * k1;
+        a[n2][k1] = cmul(a[n2][k1], mk<T>(T(cs[e][0]), T(S * cs[e][1])));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cpx<T> t[3] = {a[0][k1], a[1][k1], a[2][k1]};
+      Dft<3, S, T>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) v[k1 + 4 * k2] = t[k2];
     }
   }
 };

@@ -976,10 +976,13 @@ struct ZsArgs {
 // the first evaluation of a penalty-drift step (belastic.cpp:677-683): the
 // new state is not stored, and only P (its z-inverse) and the Parseval and
 // DC terms are.
-// Plan variant of the stage combines (see Plan): the default plan.  The
+// Plan variant of the stage combines (see Plan): the default plan, except
+// FP32 at 192, 12 x 4 x 4 (two exchanges instead of three, the same 12
+// elements per thread; 218-240 registers against 205-226, two waves per SIMD
+// either way; 4 x 192^3 batched 3.61 -> 3.57 ms per step).  At 256 the
 // 16 x 16 plan needs 248-256 registers there and ran slower (stage 4 158 ->
 // 182 us), and so did 4 x 4 x 4 x 4 over 64 threads (100-128 registers).
-template <typename T, int N> constexpr int zsplan() { return 0; }
+template <typename T, int N> constexpr int zsplan() { return (sizeof(T) == 4 && N == 192) ? 3 : 0; }
 
 // LDS bytes of a stage combine: two column tiles for the transforms, or the
 // staged coefficient table, whichever is larger
