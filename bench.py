@@ -173,6 +173,30 @@ def batched(vb, steps, seed=20260004):
                                       "not_ok": [i for i, r in enumerate(reps_s) if r["status"] != 0]}}
 
 
+def group_alone(n, steps):
+    """One size group of the batched form alone (its solves as in
+    cudabubble_amd.workloads.batch_mix), on the default kernel schedules: the
+    driver of the groups' kernel traces.  Prints one JSON line."""
+    import cudabubble_amd as vb
+    from cudabubble_amd.workloads import batch_mix
+    ps, phis = batch_mix(vb)
+    sel = [i for i, p in enumerate(ps) if p.nx == n]
+    if not sel:
+        print("the batched form has no %d^3 solves" % n, file=sys.stderr)
+        sys.exit(2)
+    b = vb.Batch([ps[i] for i in sel], [phis[i] for i in sel])
+    b.step(4)     # start-up: alpha/beta, tables, graph capture
+    sync()
+    t0 = time.perf_counter()
+    reps = b.step(steps)
+    sync()
+    dt = time.perf_counter() - t0
+    b.close()
+    print(json.dumps({"group": "%d x %d^3" % (len(sel), n), "steps": steps, "ms_per_step": 1e3 * dt / steps,
+                      "gp_steps_per_s": sum(n ** 3 * r["evaluations"] for r in reps) / dt,
+                      "not_ok": [i for i, r in zip(sel, reps) if r["status"] != 0]}), flush=True)
+
+
 def converge_run(vb, p, phi, window=50, tol=1e-6):
     """One request alone through vb_step, with the server's stopping rule:
     relative energy drop over `window` accepted steps below tol, t >= tmax or a
@@ -427,6 +451,9 @@ def main():
                     help="keep schedule 0 instead of autotuning the kernel schedules of each grid size")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (counter calibration)")
+    ap.add_argument("--group-alone", type=int, default=0, metavar="N",
+                    help="run only the batched form's N^3 size group alone for --steps steps "
+                         "(default schedules; the driver of the groups' kernel traces)")
     ap.add_argument("--driver-forms", action="store_true",
                     help="run only the per-attempt timing of the driver forms (host vs device "
                          "decisions, eager vs graph Euler attempts)")
@@ -436,6 +463,9 @@ def main():
         return
     if args.driver_forms:
         driver_forms()
+        return
+    if args.group_alone:
+        group_alone(args.group_alone, args.steps)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver path runs on a single device and "
