@@ -52,9 +52,9 @@ __device__ __forceinline__ cpx<T> rot(cpx<T> a) {
 // only where measured to pay: 256 = 16 * 16, 128 = 16 * 8 and 192 = 24 * 8
 // for the FP32 column kernels and x kernels, 512 = 16 * 8 * 4 for the FP32 x
 // kernel.
-// V = 2 goes the other way (more threads per transform, fewer registers).
 // V = 3 keeps V = 0's elements per thread with fewer passes (192 = 12 * 4 * 4,
-// the FP32 stage combines).
+// the FP32 stage combines).  (V = 2, more threads per transform and fewer
+// registers, measured slower at 256 and is not defined.)
 template <int N, int V = 0> struct Plan;
 template <> struct Plan<16, 0>  { static constexpr int E = 4,  NP = 2; static constexpr int R[2] = {4, 4}; };
 template <> struct Plan<32, 0>  { static constexpr int E = 8,  NP = 2; static constexpr int R[2] = {8, 4}; };
