@@ -178,12 +178,13 @@ def group_alone(n, steps):
     cudabubble_amd.workloads.batch_mix), on the default kernel schedules: the
     driver of the groups' kernel traces.  Prints one JSON line."""
     import cudabubble_amd as vb
-    from cudabubble_amd.workloads import batch_mix
+    from cudabubble_amd.workloads import BATCH_GROUPS, batch_mix
+    if n not in [m for m, _ in BATCH_GROUPS]:
+        print("the batched form has no %d^3 solves (its sizes: %s)" % (n, ", ".join(str(m) for m, _ in BATCH_GROUPS)),
+              file=sys.stderr)
+        sys.exit(2)
     ps, phis = batch_mix(vb)
     sel = [i for i, p in enumerate(ps) if p.nx == n]
-    if not sel:
-        print("the batched form has no %d^3 solves" % n, file=sys.stderr)
-        sys.exit(2)
     b = vb.Batch([ps[i] for i in sel], [phis[i] for i in sel])
     b.step(4)     # start-up: alpha/beta, tables, graph capture
     sync()
