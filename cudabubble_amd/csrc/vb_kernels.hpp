@@ -356,6 +356,10 @@ template <typename T, int N> constexpr int xrplan_alt() {
 // threads of a k_xrow workgroup: as many rows as the other x kernels'
 // workgroups (two per row pair), so the partial sums line up
 template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() { return 2 * xlanes<T, N>() * TFof<N, XP>(); }
+// Minimum waves per SIMD of k_xrow: three in FP32 at 256 (164 registers
+// instead of 191, no spills; 56.1 -> 54.5 us at 256^3), else no minimum (the
+// same cap at 192 spilled: 4 x 192^3 batched 3.56 -> 4.03 ms per step).
+template <typename T, int N> constexpr int xrwaves() { return (sizeof(T) == 4 && N == 256) ? 3 : 1; }
 
 // The x kernel of an ETD stage, one row per lane: the x-inverse of P + i Q of
 // the row gives v and lap v; f, tmp1 and s follow pointwise; the x-forward of
@@ -379,7 +383,7 @@ template <typename T, int N, int XP = xrplan<T, N>()> constexpr int xrthreads() 
 // integrals V, A, E_b of a field, as in FP32-mixed mode's FP64 energy of the
 // state (T = double).
 template <typename T, int N, bool TAB = false, int XP = xrplan<T, N>(), bool SUMS = false>
-__global__ void __launch_bounds__((xrthreads<T, N, XP>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
+__global__ void __launch_bounds__((xrthreads<T, N, XP>()), (xrwaves<T, N>())) k_xrow(XaArgs<T> a0, const XaArgs<T>* tab) {
   const XaArgs<T> a = TAB ? tab[blockIdx.z] : a0;
   constexpr int E = Plan<N, XP>::E, TF = N / E, RL = 2 * xlanes<T, N>(), LL = row_lds_len<N>(),
                 NT = xrthreads<T, N, XP>();
